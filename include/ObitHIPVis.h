@@ -1,0 +1,102 @@
+/*
+ * ObitHIPVis.h -- extensions of the drop-in C ABI for whole-data-set
+ * residency in HBM and batched launches.  These have no counterpart in
+ * Obit's CUDA layer: the reference boundary moves one uv buffer (<= 10,000
+ * rows, ObitGPUSkyModel.c:236-245) per call.  A caller that keeps every
+ * visibility of a field resident (Obit's MFImage/Imager major cycle) calls
+ * these instead of ObitCUDASkyModelDFTCalc / ObitCUDAGridGrid per buffer.
+ * Every function returns 0 on success or a status below, with a message
+ * from ObitHIPLastError().
+ */
+#ifndef OBITHIPVIS_H
+#define OBITHIPVIS_H
+#include <stddef.h>
+#include "ObitCUDASkyModelInfoDef.h"
+#include "ObitCUDAGridInfoDef.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OBIT_HIP_OK        0
+#define OBIT_HIP_EINVAL    1   /* bad argument or layout */
+#define OBIT_HIP_EHIP      2   /* HIP runtime error */
+#define OBIT_HIP_ENOMEM    3   /* device allocation failed */
+#define OBIT_HIP_ESTATE    4   /* call out of order (e.g. no model set) */
+
+const char *ObitHIPLastError(void);
+
+/* ---- memory helpers with size_t sizes (the reference's int sizes stop at
+ * 2 GiB, ObitCUDAUtil.h:99-152) ---- */
+int ObitHIPMalloc(void **ptr, size_t bytes);
+int ObitHIPFree(void *ptr);
+int ObitHIPMemcpyH2D(void *dst, const void *src, size_t bytes);
+int ObitHIPMemcpyD2H(void *dst, const void *src, size_t bytes);
+int ObitHIPMemcpyD2D(void *dst, const void *src, size_t bytes);
+int ObitHIPDeviceSynchronize(void);
+float ObitHIPEventElapsedMs(int *start, int *stop);
+int ObitHIPDeviceInfo(int device, char *name, int nameLen, int *nCU, int *clockKHz,
+                      long long *memBytes);
+
+/* ---- DFT sky model ---- */
+
+/* Channel frequencies in FP64 (Hz, in the freqScale order of the data,
+ * nchan*nIF values) and the uv reference frequency.  Without this call the
+ * library fits each IF's FP32 freqScale with a line in FP64; with it, the
+ * phase 2*pi*(ul+vm+wn)*nu/nu0 is formed from these exact values. */
+int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq,
+                              double uvRefFreq);
+/* Reference frequencies of the component spectra in FP64 (Hz): one value
+ * (parametrised spectra: the image reference frequency) or nSpec values
+ * (tabulated spectra: the coarse-bin frequencies).  Without this call they
+ * are derived from h_visInfo->freqRat. */
+int ObitHIPSkyModelDFTSetSpecRef(GPUInfo *gpuInfo, const double *refFreq, int n);
+
+/* Resident visibilities: copy nvis rows (lenvis floats each) to HBM, or
+ * attach a device buffer the caller owns.  CalcResident applies the model
+ * to every resident row in one persistent launch. */
+int ObitHIPSkyModelDFTSetVis(GPUInfo *gpuInfo, const float *h_vis, long long nvis);
+int ObitHIPSkyModelDFTAttachVis(GPUInfo *gpuInfo, float *d_vis, long long nvis);
+int ObitHIPSkyModelDFTGetVis(GPUInfo *gpuInfo, float *h_vis, long long nvis);
+int ObitHIPSkyModelDFTCalcResident(GPUInfo *gpuInfo);
+/* The same operation with the plain all-FP64 kernel (FP64 phase, FP64
+ * sincos per evaluation, FP64 sums): the precision/throughput comparator. */
+int ObitHIPSkyModelDFTCalcFP64(GPUInfo *gpuInfo);
+/* CalcResident captured once into a hipGraph and replayed. */
+int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo);
+/* Kernel shape: channels per thread (16 or 32), component tile (32, 64
+ * or 128), workgroups per CU of the persistent grid (1..8).  Autotune
+ * times every shape on the first sampleRows resident rows and keeps the
+ * fastest. */
+int ObitHIPSkyModelDFTSetVariant(GPUInfo *gpuInfo, int cpt, int tile, int wgPerCU);
+int ObitHIPSkyModelDFTGetVariant(GPUInfo *gpuInfo, int *cpt, int *tile, int *wgPerCU);
+int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps);
+/* Sin/cos policy: 0 hardware sin/cos for the centre phasor and the
+ * per-channel step; 1 hardware for the centre phasor, Taylor polynomial
+ * for the step (default); 2 polynomial for both. */
+int ObitHIPSkyModelDFTSetPrecision(GPUInfo *gpuInfo, int prec);
+/* hipStream_t on which the library launches for this gpuInfo. */
+void *ObitHIPSkyModelDFTStream(GPUInfo *gpuInfo);
+
+/* ---- Gridding ---- */
+
+/* Grid nvis resident rows (device pointer, lenvis floats each) into every
+ * facet on iGPU, in one binning pass and one tile-gridding launch. */
+int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis);
+/* Trim the hw guard columns and swap the v halves of every plane of a
+ * facet (ObitGPUGrid2Host + ObitCArray2DCenter) on the device:
+ * out[nplane][ny][nx/2+1][2]. */
+int ObitHIPGrid2Device(CUDAGridInfo *in, long ifacet, float *d_out);
+/* The same into host memory (out as above). */
+int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out);
+/* Per-sample gridding indices of one facet, computed by the device code
+ * the gridder uses: for row r and channel c, index r*nchan+c of iu, iv
+ * (first of the 7 rows), itu, itv (GCF block 0..100) and vis[2*(..)]. */
+int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis,
+                         int *iu, int *iv, int *itu, int *itv, float *vis);
+void *ObitHIPGridStream(CUDAGridInfo *in, long iGPU);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OBITHIPVIS_H */

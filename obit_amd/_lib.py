@@ -1,0 +1,193 @@
+"""ctypes binding of libobit_hip.so (the HIP implementation of Obit's GPU layer).
+
+The library is built in-tree (``obit_amd/libobit_hip.so``) by
+``__graft_entry__.build()`` / ``obit_amd/csrc/Makefile``.  Loading fails
+loudly when it is missing: there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libobit_hip.so")
+
+
+class HipError(RuntimeError):
+    """A non-zero status from an ObitHIP* / ObitHost* entry point."""
+
+
+# ---- plain structs of the host classes (csrc/host.h) ----
+
+class HostUVDesc(C.Structure):
+    _fields_ = [
+        ("nvis", C.c_int), ("lrec", C.c_int), ("nrparm", C.c_int),
+        ("ilocu", C.c_int), ("ilocv", C.c_int), ("ilocw", C.c_int),
+        ("iloct", C.c_int), ("ilocb", C.c_int), ("ilocsu", C.c_int),
+        ("nchan", C.c_int), ("nif", C.c_int), ("nstok", C.c_int),
+        ("freqBeforeIF", C.c_int), ("hasIF", C.c_int),
+        ("incs", C.c_int), ("incf", C.c_int), ("incif", C.c_int),
+        ("ncorr", C.c_int),
+        ("crvalStokes", C.c_float),
+        ("freq", C.c_double),
+        ("freqArr", C.POINTER(C.c_double)),
+        ("fscale", C.POINTER(C.c_float)),
+    ]
+
+
+class HostSkyModel(C.Structure):
+    _fields_ = [
+        ("modType", C.c_int), ("nSpecTerm", C.c_int),
+        ("doDivide", C.c_int), ("doReplace", C.c_int), ("doFlip", C.c_int),
+        ("factor", C.c_float), ("stokFactor", C.c_float),
+        ("startChannel", C.c_int), ("numberChannel", C.c_int),
+        ("startIF", C.c_int), ("numberIF", C.c_int),
+        ("startPoln", C.c_int), ("numberPoln", C.c_int),
+        ("isMF", C.c_int), ("nSpec", C.c_int),
+        ("specFreq", C.POINTER(C.c_double)),
+        ("specIndex", C.POINTER(C.c_int)),
+        ("imgRefFreq", C.c_double),
+        ("cuda_device", C.c_int),
+        ("useFreq64", C.c_int),
+    ]
+
+
+class HostFacet(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("cellx", C.c_float), ("celly", C.c_float),
+        ("shift", C.c_float * 3),
+        ("rotUV", C.c_float * 9),
+        ("blmax", C.c_float), ("blmin", C.c_float),
+        ("bmTaperUV", C.c_float),
+        ("doBeam", C.c_int),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_long
+_LL = C.c_longlong
+_F = C.c_float
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes)
+_PROTOS = {
+    # ObitCUDAUtil.h
+    "CUDAMagicF": (_F, []),
+    "ObitCUDACheckGPU": (_L, [_I]),
+    "ObitCUDASetGPU": (None, [_I]),
+    "ObitCUDAResetGPU": (None, []),
+    "ObitCUDADeviceSynchronize": (None, [_P]),
+    "ObitCUDAStreamCreate": (_P, []),
+    "ObitCUDAStreamDestroy": (None, [_P]),
+    "ObitCUDAEventCreate": (_P, []),
+    "ObitCUDAEventDestroy": (None, [_P]),
+    "ObitCUDAEventRecord": (None, [_P, _P]),
+    "ObitCUDAEventSynchronize": (None, [_P]),
+    "ObitCUDAUtilAllocHost": (_P, [_I]),
+    "ObitCUDAUtilFreeHost": (None, [_P]),
+    "ObitCUDAUtilAllocGPU": (_P, [_I]),
+    "ObitCUDAUtilFreeGPU": (None, [_P]),
+    "ObitCUDAUtilHost2GPU": (None, [_P, _P, _I, _P]),
+    "ObitCUDAUtilHostAny2GPU": (None, [_P, _P, _I, _P]),
+    "ObitCUDAUtilGPU2Host": (None, [_P, _P, _I, _P]),
+    "ObitCUDAUtilGPUAny2Host": (None, [_P, _P, _I, _P]),
+    "ObitCUDAUtilMemory": (C.c_size_t, [_I]),
+    "ObitCUDAUtilPointerType": (None, [_P, C.c_char_p]),
+    # ObitCUDASkyModel.h
+    "ObitCUDASkyModelDFTInit": (None, [_P]),
+    "ObitCUDASkyModelDFTSetMod": (None, [_P]),
+    "ObitCUDASkyModelDFTCalc": (None, [_P]),
+    "ObitCUDASkyModelDFTShutdown": (None, [_P]),
+    # ObitCUDAGrid.h
+    "ObitCUDAGridAlloc": (None, [_P, _L, _L, _L, _L, _L, _L, _L, _I]),
+    "ObitCUDAGridInit": (None, [_P, _L]),
+    "ObitCUDAGridGrid": (None, [_P, _L, _I]),
+    "ObitCUDAGridFlip": (None, [_P, _L]),
+    "ObitCUDAGrid2CPU": (None, [_P, _L]),
+    "ObitCUDAGridShutdown": (None, [_P, _L]),
+    # ObitHIPVis.h
+    "ObitHIPLastError": (C.c_char_p, []),
+    "ObitHIPMalloc": (_I, [C.POINTER(_P), C.c_size_t]),
+    "ObitHIPFree": (_I, [_P]),
+    "ObitHIPMemcpyH2D": (_I, [_P, _P, C.c_size_t]),
+    "ObitHIPMemcpyD2H": (_I, [_P, _P, C.c_size_t]),
+    "ObitHIPMemcpyD2D": (_I, [_P, _P, C.c_size_t]),
+    "ObitHIPDeviceSynchronize": (_I, []),
+    "ObitHIPEventElapsedMs": (_F, [_P, _P]),
+    "ObitHIPDeviceInfo": (_I, [_I, C.c_char_p, _I, _IP, _IP, C.POINTER(_LL)]),
+    "ObitHIPSkyModelDFTSetFreq": (_I, [_P, _DP, _I, C.c_double]),
+    "ObitHIPSkyModelDFTSetSpecRef": (_I, [_P, _DP, _I]),
+    "ObitHIPSkyModelDFTSetVis": (_I, [_P, _P, _LL]),
+    "ObitHIPSkyModelDFTAttachVis": (_I, [_P, _P, _LL]),
+    "ObitHIPSkyModelDFTGetVis": (_I, [_P, _P, _LL]),
+    "ObitHIPSkyModelDFTCalcResident": (_I, [_P]),
+    "ObitHIPSkyModelDFTCalcFP64": (_I, [_P]),
+    "ObitHIPSkyModelDFTGraphLaunch": (_I, [_P]),
+    "ObitHIPSkyModelDFTSetVariant": (_I, [_P, _I, _I, _I]),
+    "ObitHIPSkyModelDFTGetVariant": (_I, [_P, _IP, _IP, _IP]),
+    "ObitHIPSkyModelDFTAutotune": (_I, [_P, _LL, _I]),
+    "ObitHIPSkyModelDFTSetPrecision": (_I, [_P, _I]),
+    "ObitHIPSkyModelDFTStream": (_P, [_P]),
+    "ObitHIPGridGridDevice": (_I, [_P, _L, _P, _LL]),
+    "ObitHIPGrid2Device": (_I, [_P, _L, _P]),
+    "ObitHIPGrid2HostPlanes": (_I, [_P, _L, _P]),
+    "ObitHIPGridPrepDebug": (_I, [_P, _L, _P, _LL, _P, _P, _P, _P, _P]),
+    "ObitHIPGridStream": (_P, [_P, _L]),
+    # host classes (csrc/host.h)
+    "ObitHostDFTCreate": (_P, []),
+    "ObitHostDFTInit": (_I, [_P, C.POINTER(HostUVDesc), C.POINTER(HostSkyModel), _I]),
+    "ObitHostDFTSetMod": (_I, [_P, C.POINTER(HostSkyModel), _P, _I, _I]),
+    "ObitHostDFTCalc": (_I, [_P, _P, _LL]),
+    "ObitHostDFTShutdown": (None, [_P]),
+    "ObitHostConvFuncSph": (None, [_P]),
+    "ObitHostGridCreate": (_P, [_I, _I, _IP, C.POINTER(HostUVDesc), _I, _I, _I]),
+    "ObitHostGridSetFacet": (_I, [_P, _I, C.POINTER(HostFacet), C.POINTER(HostUVDesc), _P, _P, _P, _P]),
+    "ObitHostGridBuffer": (_I, [_P, _P, _LL]),
+    "ObitHostGridFlip": (_I, [_P]),
+    "ObitHostGrid2Host": (_I, [_P, _I, _P]),
+    "ObitHostGridShutdown": (None, [_P]),
+    "ObitHostPlanesFromBins": (None, [_I, _I, _IP, _IP, _I, _IP]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libobit_hip.so and declare every prototype; raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(make -C obit_amd/csrc); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise HipError with the library's message on a non-zero status."""
+    if rc:
+        msg = load().ObitHIPLastError()
+        raise HipError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+
+def ptr(a):
+    """ctypes void pointer to a numpy array's data (must be contiguous)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return C.c_void_p(a.ctypes.data)
+
+
+def exported_names():
+    return list(_PROTOS)

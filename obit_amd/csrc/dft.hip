@@ -1,0 +1,994 @@
+// dft.hip -- DFT sky-model evaluate/subtract/divide/replace for gfx950.
+//
+// Replaces Obit's CUDA DFT layer (reference src/ObitCUDASkyModel.cu:73-963)
+// behind the same C entry points (ObitCUDASkyModel.h) plus resident/batched
+// extensions (ObitHIPVis.h).  Semantics follow Obit's CPU DFT
+// (src/ObitSkyModel.c:2354-2857, src/ObitSkyModelMF.c:1971-2440).
+//
+// Numerics ("split precision"):
+//   * phase psi_k = u x_k + v y_k + w z_k is formed in FP64 in turns per
+//     reference-frequency wavelength (x_k = 2 pi l_k / 2 pi, exact from the
+//     FP32 table), multiplied by nu/nu0 in FP64 and range-reduced to
+//     [-1/2, 1/2] turns in FP64;
+//   * one FP32 sincos for the phasor at the centre channel of the thread's
+//     channel chunk and one for the per-channel step exp(2 pi i psi dnu/nu0);
+//   * a first-order complex recurrence walks out from the centre in both
+//     directions (at most CPT/2 steps), so every channel costs one complex
+//     multiply and one complex multiply-add;
+//   * sums run in FP32 over a component tile and are folded into FP64
+//     accumulators after every tile (two-level compensated summation).
+//
+// Work decomposition: a workgroup is 256 consecutive rows x one (IF,
+// channel chunk); each thread owns one row and CPT channels in registers.
+// Component tiles (TILE components) are staged in LDS by the workgroup
+// and read as broadcasts.  The grid is persistent: gridDim.x workgroups
+// stride over all (row block, IF, chunk) items.
+#include "common.h"
+#include "ObitCUDASkyModel.h"
+#include "ObitCUDAUtil.h"
+#include "ObitHIPVis.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace obit {
+namespace dft {
+
+constexpr int kRows = 256;  // rows (threads) per workgroup
+
+enum Op { kSub = 0, kDiv = 1, kRepl = 2 };
+
+// Kernel descriptor: plain values and device pointers only.
+struct Desc {
+  float *vis;
+  long long nrow;
+  int lrec, nrparm, ilocu, ilocv, ilocw;
+  int incs, incf, incif;  // floats
+  int chanb, nchsel, IFb, nIFsel, stokb, nstoksel;
+  int nchunk, nrowblk;
+  long long nitems;
+  const double *fbase, *fdel;  // per selected IF: nu/nu0 at chanb and per channel
+  int ncomp;
+  const double *cx, *cy, *cz;  // turns per reference wavelength
+  const float *camp;           // amplitude
+  const float *cg;             // Gaussian coefficients [3][ncomp]
+  const float *tab;            // spectral amplitudes [ncomp][nIFsel][nchpad]
+  int tab_ldk, tab_nchpad;
+  int op, doSwap;
+  float stokFactor;
+};
+
+// ---- sin/cos of 2*pi*t ----
+
+// Hardware transcendental: v_sin_f32 / v_cos_f32 take revolutions.
+__device__ __forceinline__ void sincos_turns_hw(float t, float *s, float *c) {
+  *s = __builtin_amdgcn_sinf(t);
+  *c = __builtin_amdgcn_cosf(t);
+}
+
+// Quadrant reduction t = q/4 + r (exact for |t| <= 1/2), then Taylor
+// series of sin and cos in x = 2 pi r, |x| <= pi/4, truncated after x^9
+// and x^10 (truncation error < 2e-9), and a quarter-turn rotation.
+__device__ __forceinline__ void sincos_turns_poly(float t, float *s, float *c) {
+  const float q = rintf(4.0f * t);
+  const float r = fmaf(-0.25f, q, t);
+  const int iq = ((int)q) & 3;
+  const float x = r * 6.28318530717958647692f;
+  const float x2 = x * x;
+  float ps = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+  ps = fmaf(x2, ps, 8.33333333e-3f);
+  ps = fmaf(x2, ps, -1.66666667e-1f);
+  const float sn = fmaf(x * x2, ps, x);
+  float pc = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
+  pc = fmaf(x2, pc, -1.38888889e-3f);
+  pc = fmaf(x2, pc, 4.16666667e-2f);
+  pc = fmaf(x2, pc, -0.5f);
+  const float cs = fmaf(x2, pc, 1.0f);
+  // rotate (cs, sn) by iq quarter turns
+  const bool odd = iq & 1;
+  const float a = odd ? cs : sn;   // sin of rotated angle before sign
+  const float b = odd ? sn : cs;   // cos of rotated angle before sign
+  *s = (iq == 2 || iq == 3) ? -a : a;
+  *c = (iq == 1 || iq == 2) ? -b : b;
+}
+
+template <int PREC>
+__device__ __forceinline__ void sincos_turns(float t, float *s, float *c) {
+  if (PREC == 0) sincos_turns_hw(t, s, c);
+  else sincos_turns_poly(t, s, c);
+}
+
+// Apply the model of one (row, channel) to the selected Stokes, following
+// src/ObitSkyModel.c:2794-2844.
+__device__ __forceinline__ void apply_op(const Desc &d, float *p, float mr, float mi) {
+  if (d.doSwap) { const float t = mr; mr = -mi; mi = t; }
+  float wm = 1.0f;
+  if (d.op == kDiv) {
+    wm = mr * mr + mi * mi;
+    mr = mr / wm;
+    mi = mi / wm;
+    wm = sqrtf(wm);
+  }
+  for (int s = 0; s < d.nstoksel; ++s) {
+    float *pp = p + s * d.incs;
+    const float wt = pp[2];
+    if (wt <= 0.0f && d.op != kRepl) continue;
+    if (d.op == kDiv) {
+      const float re = pp[0], im = pp[1];
+      pp[0] = mr * re + mi * im;
+      pp[1] = mr * im - mi * re;
+      pp[2] = wt * wm;
+    } else if (d.op == kRepl) {
+      pp[0] = mr;
+      pp[1] = mi;
+    } else {
+      pp[0] -= mr;
+      pp[1] -= mi;
+    }
+    mr *= d.stokFactor;
+    mi *= d.stokFactor;
+  }
+}
+
+template <int CPT, int TILE, bool GAUSS, bool SPEC, int PREC>
+__global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
+  static_assert(CPT % 2 == 0, "CPT must be even");
+  __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
+  __shared__ float s_a[TILE];
+  __shared__ float s_g[GAUSS ? 3 * TILE : 1];
+  __shared__ float s_t[SPEC ? TILE * CPT : 1];
+  const int tid = threadIdx.x;
+
+  for (long long item = blockIdx.x; item < d.nitems; item += gridDim.x) {
+    const int rb = (int)(item % d.nrowblk);
+    const int ic = (int)(item / d.nrowblk);
+    const int isel = ic / d.nchunk;
+    const int c0 = (ic - isel * d.nchunk) * CPT;
+    const long long row = (long long)rb * kRows + tid;
+    const bool valid = row < d.nrow;
+    const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
+    const float uf = valid ? rp[d.ilocu] : 0.0f;
+    const float vf = valid ? rp[d.ilocv] : 0.0f;
+    const float wf = valid ? rp[d.ilocw] : 0.0f;
+    const double u = uf, v = vf, w = wf;
+    const double fd = d.fdel[isel];
+    const double fm = d.fbase[isel] + (double)(c0 + CPT / 2) * fd;
+    const float fmf = (float)fm, fdf = (float)fd;
+    const float uu = uf * uf, vv = vf * vf, uv = uf * vf;
+
+    float accr[CPT], acci[CPT];
+    double outr[CPT], outi[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) { accr[j] = acci[j] = 0.0f; outr[j] = outi[j] = 0.0; }
+
+    for (int k0 = 0; k0 < d.ncomp; k0 += TILE) {
+      __syncthreads();
+      for (int t = tid; t < TILE; t += kRows) {
+        const int k = k0 + t;
+        const bool ok = k < d.ncomp;
+        s_x[t] = ok ? d.cx[k] : 0.0;
+        s_y[t] = ok ? d.cy[k] : 0.0;
+        s_z[t] = ok ? d.cz[k] : 0.0;
+        s_a[t] = ok ? d.camp[k] : 0.0f;
+        if (GAUSS) {
+          s_g[3 * t + 0] = ok ? d.cg[k] : 0.0f;
+          s_g[3 * t + 1] = ok ? d.cg[d.ncomp + k] : 0.0f;
+          s_g[3 * t + 2] = ok ? d.cg[2 * d.ncomp + k] : 0.0f;
+        }
+      }
+      if (SPEC) {
+        for (int t = tid; t < TILE * CPT; t += kRows) {
+          const int kk = t / CPT, cc = t - kk * CPT;
+          const int k = k0 + kk;
+          s_t[t] = (k < d.ncomp)
+                       ? d.tab[(size_t)k * d.tab_ldk + (size_t)isel * d.tab_nchpad + c0 + cc]
+                       : 0.0f;
+        }
+      }
+      __syncthreads();
+      const int kn = min(TILE, d.ncomp - k0);
+      for (int kk = 0; kk < kn; ++kk) {
+        const double psi = fma(w, s_z[kk], fma(v, s_y[kk], u * s_x[kk]));
+        double pm = psi * fm, pd = psi * fd;
+        pm -= rint(pm);
+        pd -= rint(pd);
+        float sm, cm, sd, cd;
+        sincos_turns<PREC == 2 ? 1 : 0>((float)pm, &sm, &cm);
+        sincos_turns<PREC == 0 ? 0 : 1>((float)pd, &sd, &cd);
+        const float a0 = s_a[kk];
+        float gf = a0, rf = 0.0f, rbk = 0.0f, qg = 0.0f;
+        if (GAUSS) {
+          const float G = fmaf(s_g[3 * kk + 2], uv, fmaf(s_g[3 * kk + 1], vv, s_g[3 * kk] * uu));
+          const float e0 = G * fmf * fmf;
+          const float Gd = G * fdf;
+          const bool live = e0 > -60.0f;
+          gf = live ? (SPEC ? 1.0f : a0) * __expf(e0) : 0.0f;
+          rf = live ? __expf(Gd * (2.0f * fmf + fdf)) : 0.0f;
+          rbk = live ? __expf(Gd * (fdf - 2.0f * fmf)) : 0.0f;
+          qg = live ? __expf(2.0f * Gd * fdf) : 0.0f;
+        }
+        // forward from the centre channel
+        float zr = cm, zi = sm, g = gf, r = rf;
+#pragma unroll
+        for (int j = CPT / 2; j < CPT; ++j) {
+          float amp = SPEC ? s_t[kk * CPT + j] : a0;
+          if (GAUSS) amp = SPEC ? amp * g : g;
+          accr[j] = fmaf(amp, zr, accr[j]);
+          acci[j] = fmaf(amp, zi, acci[j]);
+          if (j + 1 < CPT) {
+            const float nr = zr * cd - zi * sd;
+            const float ni = zr * sd + zi * cd;
+            zr = nr; zi = ni;
+            if (GAUSS) { g *= r; r *= qg; }
+          }
+        }
+        // backward from the channel below the centre
+        float br = cm * cd + sm * sd, bi = sm * cd - cm * sd;
+        float gb = gf * rbk, rb2 = rbk * qg;
+#pragma unroll
+        for (int j = CPT / 2 - 1; j >= 0; --j) {
+          float amp = SPEC ? s_t[kk * CPT + j] : a0;
+          if (GAUSS) amp = SPEC ? amp * gb : gb;
+          accr[j] = fmaf(amp, br, accr[j]);
+          acci[j] = fmaf(amp, bi, acci[j]);
+          if (j > 0) {
+            const float nr = br * cd + bi * sd;
+            const float ni = bi * cd - br * sd;
+            br = nr; bi = ni;
+            if (GAUSS) { gb *= rb2; rb2 *= qg; }
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        outr[j] += (double)accr[j];
+        outi[j] += (double)acci[j];
+        accr[j] = acci[j] = 0.0f;
+      }
+    }
+
+    if (valid) {
+      float *p = d.vis + row * (long long)d.lrec + d.nrparm + (d.IFb + isel) * d.incif +
+                 (d.chanb + c0) * d.incf + d.stokb * d.incs;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        if (c0 + j < d.nchsel) apply_op(d, p + j * d.incf, (float)outr[j], (float)outi[j]);
+      }
+    }
+  }
+}
+
+// Plain all-FP64 comparator: one thread per (row, channel); FP64 phase in
+// radians, FP64 sincos and FP64 sums per evaluation.
+template <bool GAUSS, bool SPEC>
+__global__ __launch_bounds__(kRows) void k_dft_fp64(Desc d) {
+  constexpr int T = 64;
+  __shared__ double s_x[T], s_y[T], s_z[T], s_a[T], s_g[3 * T];
+  const int tid = threadIdx.x;
+  const long long nitems = (long long)d.nrowblk * d.nIFsel * d.nchsel;
+  for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int rb = (int)(item % d.nrowblk);
+    const int ic = (int)(item / d.nrowblk);
+    const int isel = ic / d.nchsel;
+    const int cs = ic - isel * d.nchsel;
+    const long long row = (long long)rb * kRows + tid;
+    const bool valid = row < d.nrow;
+    const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
+    const double f = d.fbase[isel] + (double)cs * d.fdel[isel];
+    const double u = (double)(valid ? rp[d.ilocu] : 0.0f) * f;
+    const double v = (double)(valid ? rp[d.ilocv] : 0.0f) * f;
+    const double w = (double)(valid ? rp[d.ilocw] : 0.0f) * f;
+    const double twopi = 6.283185307179586476925;
+    double sr = 0.0, si = 0.0;
+    for (int k0 = 0; k0 < d.ncomp; k0 += T) {
+      __syncthreads();
+      for (int t = tid; t < T; t += kRows) {
+        const int k = k0 + t;
+        const bool ok = k < d.ncomp;
+        s_x[t] = ok ? d.cx[k] * twopi : 0.0;
+        s_y[t] = ok ? d.cy[k] * twopi : 0.0;
+        s_z[t] = ok ? d.cz[k] * twopi : 0.0;
+        double a = ok ? (double)d.camp[k] : 0.0;
+        if (SPEC && ok) a = d.tab[(size_t)k * d.tab_ldk + (size_t)isel * d.tab_nchpad + cs];
+        s_a[t] = a;
+        if (GAUSS) {
+          s_g[3 * t + 0] = ok ? (double)d.cg[k] : 0.0;
+          s_g[3 * t + 1] = ok ? (double)d.cg[d.ncomp + k] : 0.0;
+          s_g[3 * t + 2] = ok ? (double)d.cg[2 * d.ncomp + k] : 0.0;
+        }
+      }
+      __syncthreads();
+      const int kn = min(T, d.ncomp - k0);
+      for (int kk = 0; kk < kn; ++kk) {
+        double amp = s_a[kk];
+        if (GAUSS) amp *= exp(s_g[3 * kk] * u * u + s_g[3 * kk + 1] * v * v + s_g[3 * kk + 2] * u * v);
+        const double ph = s_x[kk] * u + s_y[kk] * v + s_z[kk] * w;
+        double s, c;
+        sincos(ph, &s, &c);
+        sr += amp * c;
+        si += amp * s;
+      }
+    }
+    if (valid) {
+      float *p = d.vis + row * (long long)d.lrec + d.nrparm + (d.IFb + isel) * d.incif +
+                 (d.chanb + cs) * d.incf + d.stokb * d.incs;
+      apply_op(d, p, (float)sr, (float)si);
+    }
+  }
+}
+
+// Convert Obit's component table (rows of 'size' floats) into the
+// resident layout: positions in turns (FP64), amplitude, Gaussian terms.
+// Table layouts: src/ObitSkyModel.c:1965-2043, src/ObitSkyModelMF.c:1480-1514.
+__global__ void k_prep_comps(const float *model, int size, int n, int gaussOff, double *cx,
+                             double *cy, double *cz, float *camp, float *cg) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float *r = model + (size_t)k * size;
+  const double inv2pi = 0.15915494309189533577;
+  camp[k] = r[0];
+  cx[k] = (double)r[1] * inv2pi;
+  cy[k] = (double)r[2] * inv2pi;
+  cz[k] = (double)r[3] * inv2pi;
+  if (gaussOff >= 0) {
+    cg[k] = r[gaussOff];
+    cg[n + k] = r[gaussOff + 1];
+    cg[2 * n + k] = r[gaussOff + 2];
+  }
+}
+
+// Spectral amplitude per (component, selected IF, selected channel) in
+// FP64, stored FP32.  Parametrised (type Spec): a*exp(sum_t c_t L^(t+1)),
+// L = ln(nu/nu_ref), coefficients at coefOff (src/ObitSkyModel.c:2528-2552,
+// CPU sign).  Tabulated (TSpec): S_bin*exp(alpha*L), L = ln(nu/nu_bin),
+// alpha at coefOff and S at coefOff+1+bin (src/ObitSkyModelMF.c:2143-2172).
+__global__ void k_prep_spec(const float *model, int size, int n, int tabulated, int coefOff,
+                            int nterm, const double *lnr, const int *bin, int nIFsel,
+                            int nchsel, int nchpad, float *tab) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = (long long)nIFsel * nchpad;
+  if (idx >= (long long)n * per) return;
+  const int k = (int)(idx / per);
+  const int rem = (int)(idx - (long long)k * per);
+  const int isel = rem / nchpad, cs = rem - isel * nchpad;
+  float out = 0.0f;
+  if (cs < nchsel) {
+    const float *r = model + (size_t)k * size;
+    const double L = lnr[isel * nchsel + cs];
+    if (tabulated) {
+      const int b = bin[isel * nchsel + cs];
+      out = (float)((double)r[coefOff + 1 + b] * exp((double)r[coefOff] * L));
+    } else {
+      double a = 0.0, Lp = L;
+      for (int t = 0; t < nterm; ++t) { a += (double)r[coefOff + t] * Lp; Lp *= L; }
+      out = (float)((double)r[0] * exp(a));
+    }
+  }
+  tab[idx] = out;
+}
+
+// ---------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------
+
+using KernelFn = void (*)(Desc);
+
+template <int CPT, int TILE, bool G, bool S, int P>
+KernelFn pick1() { return k_dft_split<CPT, TILE, G, S, P>; }
+
+template <int CPT, int TILE, int P>
+KernelFn pick_gs(bool g, bool s) {
+  if (g) return s ? pick1<CPT, TILE, true, true, P>() : pick1<CPT, TILE, true, false, P>();
+  return s ? pick1<CPT, TILE, false, true, P>() : pick1<CPT, TILE, false, false, P>();
+}
+
+template <int P>
+KernelFn pick_shape(int cpt, int tile, bool g, bool s) {
+  if (cpt == 16) {
+    if (tile == 32) return pick_gs<16, 32, P>(g, s);
+    if (tile == 128) return pick_gs<16, 128, P>(g, s);
+    return pick_gs<16, 64, P>(g, s);
+  }
+  if (tile == 32) return pick_gs<32, 32, P>(g, s);
+  if (tile == 128) return pick_gs<32, 128, P>(g, s);
+  return pick_gs<32, 64, P>(g, s);
+}
+
+KernelFn pick_split(int cpt, int tile, bool g, bool s, int prec) {
+  if (prec == 0) return pick_shape<0>(cpt, tile, g, s);
+  if (prec == 2) return pick_shape<2>(cpt, tile, g, s);
+  return pick_shape<1>(cpt, tile, g, s);
+}
+
+KernelFn pick_fp64(bool g, bool s) {
+  if (g) return s ? k_dft_fp64<true, true> : k_dft_fp64<true, false>;
+  return s ? k_dft_fp64<false, true> : k_dft_fp64<false, false>;
+}
+
+struct State {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nCU = 256;
+  // model
+  bool modelReady = false;
+  int type = 0, nterm = 0, size = 0, ncomp = 0;
+  double *cx = nullptr, *cy = nullptr, *cz = nullptr;
+  float *camp = nullptr, *cg = nullptr;
+  int capComp = 0;
+  const float *modelDev = nullptr;  // caller's table, for the spectral prep
+  // spectral table
+  float *tab = nullptr;
+  size_t tabCap = 0;
+  bool tabReady = false;
+  long long tabKey = -1;
+  // frequencies
+  std::vector<double> freq64;
+  double uvRef = 0.0;
+  std::vector<double> specRef;
+  std::vector<float> fscaleHost;  // cached copy of freqScale
+  double *dfreq = nullptr;        // [2*nIFsel]: fbase, fdel
+  double *hfreq = nullptr;        // pinned staging
+  int freqCap = 0;
+  bool uniform = true;
+  // resident data
+  float *dvis = nullptr;
+  long long nres = 0;
+  bool ownVis = false;
+  float *stage = nullptr;
+  size_t stageCap = 0;
+  // kernel shape
+  int cpt = 32, tile = 64, wgPerCU = 2, prec = 1;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graphKey = 0;
+};
+
+std::mutex g_mu;
+std::unordered_map<const GPUInfo *, State *> g_states;
+
+State *find_state(const GPUInfo *g) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_states.find(g);
+  return it == g_states.end() ? nullptr : it->second;
+}
+
+int model_gauss_off(int type) {
+  return (type == GPUModTypeGauss || type == GPUModTypeGaussSpec || type == GPUModTypeGaussTSpec) ? 4 : -1;
+}
+bool model_is_spec(int type) {
+  return type == GPUModTypePointSpec || type == GPUModTypePointTSpec ||
+         type == GPUModTypeGaussSpec || type == GPUModTypeGaussTSpec;
+}
+bool model_is_tab(int type) { return type == GPUModTypePointTSpec || type == GPUModTypeGaussTSpec; }
+int model_coef_off(int type) {
+  switch (type) {
+    case GPUModTypePointSpec: return 4;
+    case GPUModTypeGaussSpec: return 7;
+    case GPUModTypePointTSpec: return 4;
+    case GPUModTypeGaussTSpec: return 7;
+    default: return -1;
+  }
+}
+
+void destroy_graph(State *st) {
+  if (st->gexec) { (void)hipGraphExecDestroy(st->gexec); st->gexec = nullptr; }
+  if (st->graph) { (void)hipGraphDestroy(st->graph); st->graph = nullptr; }
+  st->graphKey = 0;
+}
+
+// Frequency of every selected channel relative to nu0, as a line per IF:
+// f(c) = fbase + (c - chanb) * fdel.  Exact FP64 values when the caller
+// gave them, else a least-squares line through the FP32 freqScale.
+int frequency_lines(State *st, const GPUVisInfo *vi, std::vector<double> &fb,
+                    std::vector<double> &fdl, std::vector<double> *perChan) {
+  const int nchsel = vi->chane - vi->chanb + 1, nIFsel = vi->IFe - vi->IFb + 1;
+  fb.assign(nIFsel, 0.0);
+  fdl.assign(nIFsel, 0.0);
+  if (perChan) perChan->assign((size_t)nIFsel * nchsel, 0.0);
+  const int nfreq = vi->nchan * vi->nIF;
+  std::vector<double> f(nfreq);
+  if (!st->freq64.empty() && st->uvRef > 0.0) {
+    if ((int)st->freq64.size() < nfreq)
+      return fail(kInvalid, "frequency table has %zu values, need %d", st->freq64.size(), nfreq);
+    for (int i = 0; i < nfreq; ++i) f[i] = st->freq64[i] / st->uvRef;
+  } else {
+    if (st->fscaleHost.size() < (size_t)nfreq) {
+      if (!vi->freqScale) return fail(kInvalid, "no frequency information (freqScale is NULL)");
+      st->fscaleHost.resize(nfreq);
+      OBIT_HIP_TRY(hipMemcpy(st->fscaleHost.data(), vi->freqScale, sizeof(float) * nfreq,
+                             hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < nfreq; ++i) f[i] = st->fscaleHost[i];
+  }
+  st->uniform = true;
+  for (int is = 0; is < nIFsel; ++is) {
+    const int IF = vi->IFb + is;
+    std::vector<double> y(nchsel);
+    for (int c = 0; c < nchsel; ++c) y[c] = f[IF * vi->kincif + (vi->chanb + c) * vi->kincf];
+    double a = y[0], b = 0.0;
+    if (nchsel > 1) {
+      if (!st->freq64.empty() && st->uvRef > 0.0) {
+        b = (y[nchsel - 1] - y[0]) / (nchsel - 1);
+      } else {  // least squares through FP32 values
+        double sx = 0, sy = 0, sxx = 0, sxy = 0;
+        for (int c = 0; c < nchsel; ++c) { sx += c; sy += y[c]; sxx += (double)c * c; sxy += c * y[c]; }
+        const double n = nchsel;
+        b = (n * sxy - sx * sy) / (n * sxx - sx * sx);
+        a = (sy - b * sx) / n;
+      }
+      for (int c = 0; c < nchsel; ++c) {
+        const double dev = std::fabs(y[c] - (a + b * c));
+        const double tol = (!st->freq64.empty() && st->uvRef > 0.0) ? 1e-12 * std::fabs(y[c])
+                                                                    : 4e-7 * std::fabs(y[c]);
+        if (dev > tol) st->uniform = false;
+      }
+    }
+    fb[is] = a;
+    fdl[is] = b;
+    if (perChan)
+      for (int c = 0; c < nchsel; ++c) (*perChan)[(size_t)is * nchsel + c] = y[c];
+  }
+  return kOK;
+}
+
+// Build (or reuse) the spectral amplitude table for the current selection.
+int ensure_spec_table(State *st, const GPUInfo *gi) {
+  const GPUVisInfo *vi = gi->h_visInfo;
+  const GPUModelInfo *mi = gi->h_modelInfo;
+  const int nchsel = vi->chane - vi->chanb + 1, nIFsel = vi->IFe - vi->IFb + 1;
+  const int nchpad = (nchsel + 63) / 64 * 64;
+  const long long key = ((long long)vi->chanb << 40) ^ ((long long)vi->chane << 28) ^
+                        ((long long)vi->IFb << 16) ^ (long long)vi->IFe;
+  if (st->tabReady && st->tabKey == key) return kOK;
+  std::vector<double> fb, fdl, fch;
+  int rc = frequency_lines(st, vi, fb, fdl, &fch);
+  if (rc) return rc;
+  const bool tabulated = model_is_tab(st->type);
+  const int coefOff = model_coef_off(st->type);
+  if (!tabulated && coefOff + st->nterm > st->size)
+    return fail(kInvalid, "%d spectral terms do not fit a %d-float component row", st->nterm, st->size);
+  std::vector<double> lnr((size_t)nIFsel * nchsel);
+  std::vector<int> bin((size_t)nIFsel * nchsel, 0);
+  for (int is = 0; is < nIFsel; ++is) {
+    for (int c = 0; c < nchsel; ++c) {
+      const int IF = vi->IFb + is, ch = vi->chanb + c;
+      const int ifq = IF * vi->kincif + ch * vi->kincf;
+      const double fr = fch[(size_t)is * nchsel + c];  // nu/nu_uv
+      double ratio;                                     // nu/nu_spectrum
+      if (tabulated) {
+        const int b = mi->specIndex ? mi->specIndex[ifq] : 0;
+        if (b < 0 || coefOff + 1 + b >= st->size)
+          return fail(kInvalid, "coarse bin %d outside the %d-float component row", b, st->size);
+        bin[(size_t)is * nchsel + c] = b;
+        if ((int)st->specRef.size() > b && st->uvRef > 0.0) ratio = fr * st->uvRef / st->specRef[b];
+        else if (vi->freqRat) ratio = fr * (double)vi->freqRat[b];
+        else return fail(kInvalid, "tabulated spectrum needs reference frequencies");
+      } else {
+        if (!st->specRef.empty() && st->uvRef > 0.0) ratio = fr * st->uvRef / st->specRef[0];
+        else if (vi->freqRat) ratio = fr * (double)vi->freqRat[0];
+        else ratio = fr;
+      }
+      lnr[(size_t)is * nchsel + c] = std::log(ratio);
+    }
+  }
+  const size_t need = (size_t)st->ncomp * nIFsel * nchpad;
+  if (need > st->tabCap) {
+    if (st->tab) OBIT_HIP_TRY(hipFree(st->tab));
+    st->tab = nullptr;
+    hipError_t e = hipMalloc(&st->tab, need * sizeof(float));
+    if (e != hipSuccess) return fail(kNoMemory, "spectral table (%zu floats): %s", need, hipGetErrorString(e));
+    st->tabCap = need;
+  }
+  double *dl = nullptr;
+  int *db = nullptr;
+  OBIT_HIP_TRY(hipMalloc(&dl, lnr.size() * sizeof(double)));
+  OBIT_HIP_TRY(hipMalloc(&db, bin.size() * sizeof(int)));
+  OBIT_HIP_TRY(hipMemcpy(dl, lnr.data(), lnr.size() * sizeof(double), hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMemcpy(db, bin.data(), bin.size() * sizeof(int), hipMemcpyHostToDevice));
+  const long long total = (long long)st->ncomp * nIFsel * nchpad;
+  if (total > 0) {
+    const int nb = (int)((total + 255) / 256);
+    hipLaunchKernelGGL(k_prep_spec, dim3(nb), dim3(256), 0, st->stream, st->modelDev, st->size,
+                       st->ncomp, tabulated ? 1 : 0, model_coef_off(st->type), st->nterm, dl, db,
+                       nIFsel, nchsel, nchpad, st->tab);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  OBIT_HIP_TRY(hipStreamSynchronize(st->stream));
+  OBIT_HIP_TRY(hipFree(dl));
+  OBIT_HIP_TRY(hipFree(db));
+  st->tabReady = true;
+  st->tabKey = key;
+  return kOK;
+}
+
+// Fill a kernel descriptor for nrow rows at vis; uploads the per-IF lines.
+int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt, Desc &d) {
+  const GPUVisInfo *vi = gi->h_visInfo;
+  const GPUModelInfo *mi = gi->h_modelInfo;
+  if (!vi || !mi) return fail(kState, "h_visInfo / h_modelInfo not set");
+  if (!st->modelReady) return fail(kState, "no model: call ObitCUDASkyModelDFTSetMod first");
+  std::memset(&d, 0, sizeof(d));
+  d.vis = vis;
+  d.nrow = nrow;
+  d.lrec = vi->lenvis;
+  d.nrparm = vi->nrparm;
+  d.ilocu = vi->ilocu; d.ilocv = vi->ilocv; d.ilocw = vi->ilocw;
+  d.incs = 3 * vi->incs; d.incf = 3 * vi->incf; d.incif = 3 * vi->incif;
+  d.chanb = vi->chanb; d.nchsel = vi->chane - vi->chanb + 1;
+  d.IFb = vi->IFb; d.nIFsel = vi->IFe - vi->IFb + 1;
+  d.stokb = vi->stokb; d.nstoksel = vi->stoke - vi->stokb + 1;
+  if (d.nchsel < 1 || d.nIFsel < 1 || d.nstoksel < 1)
+    return fail(kInvalid, "empty channel/IF/Stokes selection");
+  if (d.chanb < 0 || vi->chane >= vi->nchan || d.IFb < 0 || vi->IFe >= vi->nIF || d.stokb < 0 ||
+      vi->stoke >= vi->nstok)
+    return fail(kInvalid, "selection outside the data axes");
+  if (vi->lenvis != gi->lenvis || d.lrec <= 0)
+    return fail(kInvalid, "lenvis mismatch (%d vs %d)", vi->lenvis, gi->lenvis);
+  {
+    // the last correlation touched must lie inside the row
+    const long long last = (long long)d.nrparm + (long long)vi->IFe * d.incif +
+                           (long long)vi->chane * d.incf + (long long)vi->stoke * d.incs + 2;
+    const long long uvwMax = std::max(d.ilocu, std::max(d.ilocv, d.ilocw));
+    if (last >= d.lrec || uvwMax >= d.lrec || std::min(d.ilocu, std::min(d.ilocv, d.ilocw)) < 0)
+      return fail(kInvalid, "uv layout exceeds the %d-float row", d.lrec);
+  }
+  d.nchunk = (d.nchsel + cpt - 1) / cpt;
+  d.nrowblk = (int)((nrow + kRows - 1) / kRows);
+  d.nitems = (long long)d.nrowblk * d.nIFsel * d.nchunk;
+  // per-IF frequency lines
+  std::vector<double> fb, fdl;
+  int rc = frequency_lines(st, vi, fb, fdl, nullptr);
+  if (rc) return rc;
+  if (d.nIFsel > st->freqCap) {
+    if (st->dfreq) OBIT_HIP_TRY(hipFree(st->dfreq));
+    if (st->hfreq) OBIT_HIP_TRY(hipHostFree(st->hfreq));
+    OBIT_HIP_TRY(hipMalloc(&st->dfreq, sizeof(double) * 2 * d.nIFsel));
+    OBIT_HIP_TRY(hipHostMalloc(&st->hfreq, sizeof(double) * 2 * d.nIFsel, hipHostMallocDefault));
+    st->freqCap = d.nIFsel;
+  }
+  OBIT_HIP_TRY(hipStreamSynchronize(st->stream));  // staging buffer reuse
+  for (int i = 0; i < d.nIFsel; ++i) { st->hfreq[i] = fb[i]; st->hfreq[d.nIFsel + i] = fdl[i]; }
+  OBIT_HIP_TRY(hipMemcpyAsync(st->dfreq, st->hfreq, sizeof(double) * 2 * d.nIFsel,
+                              hipMemcpyHostToDevice, st->stream));
+  d.fbase = st->dfreq;
+  d.fdel = st->dfreq + d.nIFsel;
+  d.ncomp = st->ncomp;
+  d.cx = st->cx; d.cy = st->cy; d.cz = st->cz;
+  d.camp = st->camp; d.cg = st->cg;
+  if (model_is_spec(st->type)) {
+    rc = ensure_spec_table(st, gi);
+    if (rc) return rc;
+    d.tab = st->tab;
+    d.tab_nchpad = (d.nchsel + 63) / 64 * 64;
+    d.tab_ldk = d.nIFsel * d.tab_nchpad;
+  }
+  d.op = (int)mi->opType;
+  if (d.op < 0 || d.op > 2) return fail(kInvalid, "unknown operation %d", d.op);
+  d.doSwap = mi->doSwap;
+  d.stokFactor = mi->stokFact[1];
+  return kOK;
+}
+
+int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU) {
+  if (d.nitems <= 0) return kOK;
+  const bool g = model_gauss_off(st->type) >= 0, s = model_is_spec(st->type);
+  KernelFn fn = pick_split(cpt, tile, g, s, st->prec);
+  long long grid = (long long)st->nCU * wgPerCU;
+  if (grid > d.nitems) grid = d.nitems;
+  Desc dd = d;
+  void *args[] = {&dd};
+  OBIT_HIP_TRY(hipLaunchKernel((const void *)fn, dim3((unsigned)grid), dim3(kRows), args, 0, st->stream));
+  return kOK;
+}
+
+int launch_fp64(State *st, const Desc &d) {
+  const long long nitems = (long long)d.nrowblk * d.nIFsel * d.nchsel;
+  if (nitems <= 0) return kOK;
+  const bool g = model_gauss_off(st->type) >= 0, s = model_is_spec(st->type);
+  long long grid = (long long)st->nCU * 4;
+  if (grid > nitems) grid = nitems;
+  Desc dd = d;
+  void *args[] = {&dd};
+  OBIT_HIP_TRY(hipLaunchKernel((const void *)pick_fp64(g, s), dim3((unsigned)grid), dim3(kRows), args, 0,
+                               st->stream));
+  return kOK;
+}
+
+// Run the operation over nrow rows at a device pointer.
+int run_device(State *st, const GPUInfo *gi, float *vis, long long nrow, bool fp64) {
+  Desc d;
+  int rc = make_desc(st, gi, vis, nrow, st->cpt, d);
+  if (rc) return rc;
+  if (fp64 || !st->uniform) return launch_fp64(st, d);
+  return launch_split(st, d, st->cpt, st->tile, st->wgPerCU);
+}
+
+}  // namespace dft
+}  // namespace obit
+
+using obit::dft::State;
+using obit::dft::find_state;
+
+extern "C" {
+
+void ObitCUDASkyModelDFTInit(GPUInfo *gpuInfo) {
+  if (!gpuInfo) return;
+  State *st = find_state(gpuInfo);
+  if (!st) {
+    st = new State();
+    std::lock_guard<std::mutex> lk(obit::dft::g_mu);
+    obit::dft::g_states[gpuInfo] = st;
+  }
+  st->device = gpuInfo->cuda_device < 0 ? 0 : gpuInfo->cuda_device;
+  OBIT_HIP_CHECK(hipSetDevice(st->device));
+  if (!st->stream) OBIT_HIP_CHECK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  OBIT_HIP_CHECK(hipGetDeviceProperties(&prop, st->device));
+  st->nCU = prop.multiProcessorCount;
+  st->fscaleHost.clear();
+}
+
+void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
+  State *st = find_state(gpuInfo);
+  if (!st) { ObitCUDASkyModelDFTInit(gpuInfo); st = find_state(gpuInfo); }
+  const GPUModelInfo *mi = gpuInfo->h_modelInfo;
+  if (!mi) { std::fprintf(stderr, "obit_hip: DFTSetMod without h_modelInfo\n"); std::abort(); }
+  OBIT_HIP_CHECK(hipSetDevice(st->device));
+  const int n = mi->nmodel[0], size = mi->size;
+  if (n < 0 || size < 4) { std::fprintf(stderr, "obit_hip: bad model shape %d x %d\n", n, size); std::abort(); }
+  st->type = (int)mi->type;
+  st->nterm = mi->nterm;
+  st->size = size;
+  st->ncomp = n;
+  st->modelDev = mi->model[0];
+  const int gOff = obit::dft::model_gauss_off(st->type);
+  if (gOff >= 0 && size < 7) { std::fprintf(stderr, "obit_hip: Gaussian model needs 7 floats per row\n"); std::abort(); }
+  if (n > st->capComp) {
+    (void)hipFree(st->cx); (void)hipFree(st->cy); (void)hipFree(st->cz); (void)hipFree(st->camp); (void)hipFree(st->cg);
+    const size_t m = (size_t)std::max(n, 1);
+    OBIT_HIP_CHECK(hipMalloc(&st->cx, m * sizeof(double)));
+    OBIT_HIP_CHECK(hipMalloc(&st->cy, m * sizeof(double)));
+    OBIT_HIP_CHECK(hipMalloc(&st->cz, m * sizeof(double)));
+    OBIT_HIP_CHECK(hipMalloc(&st->camp, m * sizeof(float)));
+    OBIT_HIP_CHECK(hipMalloc(&st->cg, 3 * m * sizeof(float)));
+    st->capComp = n;
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(obit::dft::k_prep_comps, dim3((n + 255) / 256), dim3(256), 0, st->stream,
+                       mi->model[0], size, n, gOff, st->cx, st->cy, st->cz, st->camp, st->cg);
+    OBIT_HIP_CHECK(hipGetLastError());
+  }
+  OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+  st->modelReady = true;
+  st->tabReady = false;
+  obit::dft::destroy_graph(st);
+}
+
+void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
+  State *st = find_state(gpuInfo);
+  if (!st || !st->modelReady) {
+    std::fprintf(stderr, "obit_hip: DFTCalc before DFTInit/DFTSetMod\n");
+    std::abort();
+  }
+  OBIT_HIP_CHECK(hipSetDevice(st->device));
+  const long long nvis = gpuInfo->nvis;
+  const size_t bytes = (size_t)nvis * gpuInfo->lenvis * sizeof(float);
+  if (nvis <= 0) return;
+  if (bytes > st->stageCap) {
+    if (st->stage) OBIT_HIP_CHECK(hipFree(st->stage));
+    OBIT_HIP_CHECK(hipMalloc(&st->stage, bytes));
+    st->stageCap = bytes;
+  }
+  OBIT_HIP_CHECK(hipMemcpyAsync(st->stage, gpuInfo->h_data, bytes, hipMemcpyHostToDevice, st->stream));
+  int rc = obit::dft::run_device(st, gpuInfo, st->stage, nvis, false);
+  if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
+  OBIT_HIP_CHECK(hipMemcpyAsync(gpuInfo->h_data, st->stage, bytes, hipMemcpyDeviceToHost, st->stream));
+  OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
+  State *st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(obit::dft::g_mu);
+    auto it = obit::dft::g_states.find(gpuInfo);
+    if (it != obit::dft::g_states.end()) { st = it->second; obit::dft::g_states.erase(it); }
+  }
+  if (!st) return;
+  (void)hipSetDevice(st->device);
+  (void)hipStreamSynchronize(st->stream);
+  obit::dft::destroy_graph(st);
+  (void)hipFree(st->cx); (void)hipFree(st->cy); (void)hipFree(st->cz); (void)hipFree(st->camp); (void)hipFree(st->cg);
+  (void)hipFree(st->tab); (void)hipFree(st->dfreq); (void)hipFree(st->stage);
+  if (st->hfreq) (void)hipHostFree(st->hfreq);
+  if (st->ownVis) (void)hipFree(st->dvis);
+  (void)hipStreamDestroy(st->stream);
+  delete st;
+}
+
+// ---- extensions ----
+
+int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq, double uvRefFreq) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTSetFreq before DFTInit");
+  if (!freqArr || nfreq <= 0 || !(uvRefFreq > 0.0)) return obit::fail(obit::kInvalid, "bad frequency table");
+  st->freq64.assign(freqArr, freqArr + nfreq);
+  st->uvRef = uvRefFreq;
+  st->tabReady = false;
+  obit::dft::destroy_graph(st);
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTSetSpecRef(GPUInfo *gpuInfo, const double *refFreq, int n) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTSetSpecRef before DFTInit");
+  if (!refFreq || n <= 0) return obit::fail(obit::kInvalid, "bad reference frequencies");
+  st->specRef.assign(refFreq, refFreq + n);
+  st->tabReady = false;
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTSetVis(GPUInfo *gpuInfo, const float *h_vis, long long nvis) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTSetVis before DFTInit");
+  if (nvis < 0 || (nvis > 0 && !h_vis)) return obit::fail(obit::kInvalid, "bad visibility buffer");
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  const size_t bytes = (size_t)nvis * gpuInfo->lenvis * sizeof(float);
+  if (st->ownVis && st->dvis) OBIT_HIP_TRY(hipFree(st->dvis));
+  st->dvis = nullptr;
+  st->ownVis = true;
+  hipError_t e = hipMalloc(&st->dvis, bytes ? bytes : 4);
+  if (e != hipSuccess) return obit::fail(obit::kNoMemory, "resident vis (%zu B): %s", bytes, hipGetErrorString(e));
+  OBIT_HIP_TRY(hipMemcpy(st->dvis, h_vis, bytes, hipMemcpyHostToDevice));
+  st->nres = nvis;
+  obit::dft::destroy_graph(st);
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTAttachVis(GPUInfo *gpuInfo, float *d_vis, long long nvis) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTAttachVis before DFTInit");
+  if (nvis < 0 || (nvis > 0 && !d_vis)) return obit::fail(obit::kInvalid, "bad device buffer");
+  if (st->ownVis && st->dvis) OBIT_HIP_TRY(hipFree(st->dvis));
+  st->dvis = d_vis;
+  st->ownVis = false;
+  st->nres = nvis;
+  obit::dft::destroy_graph(st);
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTGetVis(GPUInfo *gpuInfo, float *h_vis, long long nvis) {
+  State *st = find_state(gpuInfo);
+  if (!st || !st->dvis) return obit::fail(obit::kState, "no resident visibilities");
+  if (nvis > st->nres) return obit::fail(obit::kInvalid, "asked for %lld rows, %lld resident", nvis, st->nres);
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  OBIT_HIP_TRY(hipStreamSynchronize(st->stream));
+  OBIT_HIP_TRY(hipMemcpy(h_vis, st->dvis, (size_t)nvis * gpuInfo->lenvis * sizeof(float), hipMemcpyDeviceToHost));
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTCalcResident(GPUInfo *gpuInfo) {
+  State *st = find_state(gpuInfo);
+  if (!st || !st->dvis) return obit::fail(obit::kState, "no resident visibilities");
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  return obit::dft::run_device(st, gpuInfo, st->dvis, st->nres, false);
+}
+
+int ObitHIPSkyModelDFTCalcFP64(GPUInfo *gpuInfo) {
+  State *st = find_state(gpuInfo);
+  if (!st || !st->dvis) return obit::fail(obit::kState, "no resident visibilities");
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  return obit::dft::run_device(st, gpuInfo, st->dvis, st->nres, true);
+}
+
+int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo) {
+  State *st = find_state(gpuInfo);
+  if (!st || !st->dvis) return obit::fail(obit::kState, "no resident visibilities");
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  const long long key = (long long)(size_t)st->dvis ^ (st->nres << 20) ^ ((long long)st->cpt << 8) ^
+                        ((long long)st->tile << 12) ^ ((long long)st->wgPerCU << 4) ^ st->prec;
+  if (!st->gexec || st->graphKey != key) {
+    obit::dft::destroy_graph(st);
+    obit::dft::Desc d;
+    int rc = obit::dft::make_desc(st, gpuInfo, st->dvis, st->nres, st->cpt, d);
+    if (rc) return rc;
+    OBIT_HIP_TRY(hipStreamSynchronize(st->stream));
+    OBIT_HIP_TRY(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    rc = st->uniform ? obit::dft::launch_split(st, d, st->cpt, st->tile, st->wgPerCU)
+                     : obit::dft::launch_fp64(st, d);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return obit::fail(obit::kHipError, "graph capture: %s", hipGetErrorString(e));
+    st->graph = g;
+    OBIT_HIP_TRY(hipGraphInstantiate(&st->gexec, st->graph, nullptr, nullptr, 0));
+    st->graphKey = key;
+  }
+  OBIT_HIP_TRY(hipGraphLaunch(st->gexec, st->stream));
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTSetVariant(GPUInfo *gpuInfo, int cpt, int tile, int wgPerCU) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTSetVariant before DFTInit");
+  if (cpt != 16 && cpt != 32) return obit::fail(obit::kInvalid, "cpt must be 16 or 32");
+  if (tile != 32 && tile != 64 && tile != 128) return obit::fail(obit::kInvalid, "tile must be 32, 64 or 128");
+  if (wgPerCU < 1 || wgPerCU > 8) return obit::fail(obit::kInvalid, "wgPerCU must be 1..8");
+  st->cpt = cpt; st->tile = tile; st->wgPerCU = wgPerCU;
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTGetVariant(GPUInfo *gpuInfo, int *cpt, int *tile, int *wgPerCU) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTGetVariant before DFTInit");
+  if (cpt) *cpt = st->cpt;
+  if (tile) *tile = st->tile;
+  if (wgPerCU) *wgPerCU = st->wgPerCU;
+  return obit::kOK;
+}
+
+// Precision policy of the per-channel step: 0 hardware sin/cos for both
+// phasors, 1 hardware for the centre phasor and the polynomial for the
+// step (default), 2 polynomial for both.
+int ObitHIPSkyModelDFTSetPrecision(GPUInfo *gpuInfo, int prec) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTSetPrecision before DFTInit");
+  if (prec < 0 || prec > 2) return obit::fail(obit::kInvalid, "precision policy must be 0..2");
+  st->prec = prec;
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps) {
+  State *st = find_state(gpuInfo);
+  if (!st || !st->dvis) return obit::fail(obit::kState, "autotune needs resident visibilities");
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  long long n = std::min(std::max(sampleRows, 1LL), st->nres);
+  if (reps < 1) reps = 1;
+  const size_t bytes = (size_t)n * gpuInfo->lenvis * sizeof(float);
+  float *tmp = nullptr;
+  OBIT_HIP_TRY(hipMalloc(&tmp, bytes));
+  OBIT_HIP_TRY(hipMemcpy(tmp, st->dvis, bytes, hipMemcpyDeviceToDevice));
+  hipEvent_t e0, e1;
+  OBIT_HIP_TRY(hipEventCreate(&e0));
+  OBIT_HIP_TRY(hipEventCreate(&e1));
+  const int cpts[2] = {16, 32}, tiles[3] = {32, 64, 128}, wgs[3] = {1, 2, 4};
+  float best = 1e30f;
+  int bc = st->cpt, bt = st->tile, bw = st->wgPerCU;
+  int rc = obit::kOK;
+  for (int ci = 0; ci < 2 && !rc; ++ci) {
+    obit::dft::Desc d;
+    rc = obit::dft::make_desc(st, gpuInfo, tmp, n, cpts[ci], d);
+    if (rc) break;
+    for (int ti = 0; ti < 3 && !rc; ++ti) {
+      for (int wi = 0; wi < 3 && !rc; ++wi) {
+        rc = obit::dft::launch_split(st, d, cpts[ci], tiles[ti], wgs[wi]);  // warm
+        if (rc) break;
+        (void)hipEventRecord(e0, st->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = obit::dft::launch_split(st, d, cpts[ci], tiles[ti], wgs[wi]);
+        (void)hipEventRecord(e1, st->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; bc = cpts[ci]; bt = tiles[ti]; bw = wgs[wi]; }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(tmp);
+  if (rc) return rc;
+  st->cpt = bc; st->tile = bt; st->wgPerCU = bw;
+  obit::dft::destroy_graph(st);
+  return obit::kOK;
+}
+
+void *ObitHIPSkyModelDFTStream(GPUInfo *gpuInfo) {
+  State *st = find_state(gpuInfo);
+  return st ? (void *)st->stream : nullptr;
+}
+
+}  // extern "C"

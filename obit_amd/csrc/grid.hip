@@ -1,0 +1,932 @@
+// grid.hip -- convolutional uv gridding for gfx950.
+//
+// Replaces Obit's CUDA gridder (reference src/ObitCUDAGrid.cu:90-920) behind
+// the same C entry points (ObitCUDAGrid.h) plus resident extensions
+// (ObitHIPVis.h).  Per-sample arithmetic restates Obit's CPU gridder
+// (src/ObitThreadGrid.c:1263-1387 fast_prep_grid, :2045-2177 fast_rot,
+// :2002-2027 fast_grid) operation by operation with FP contraction off, so
+// cell indices, GCF taps and weighted sample values are bit-identical to
+// the CPU's (the FP32 sincos of the facet phase shift is the same Cephes
+// sequence the CPU's SSE build evaluates).
+//
+// Accumulation: every facet-plane grid is cut into TU x TV-cell tiles.
+// A binning pass lists, for each tile, the (row, plane) tracks whose 7x7
+// footprints can touch it (the footprint of a track is bounded by the
+// samples at the plane's lowest and highest frequency, since cells are
+// monotonic in frequency).  One workgroup per non-empty tile accumulates
+// its tracks in LDS with LDS float atomics and then adds the tile to the
+// grid in HBM with plain loads/stores: no cell is written by two
+// workgroups, so there are no global atomics.
+#include "common.h"
+#include "ObitCUDAGrid.h"
+#include "ObitCUDAUtil.h"
+#include "ObitHIPVis.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace obit {
+namespace grid {
+
+constexpr int TU = 64;        // tile width in grid columns (complex cells)
+constexpr int TV = 32;        // tile height in grid rows
+constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
+constexpr int kThreads = 256;
+
+struct FacetDev {
+  float *grid;
+  int sizeGrid;  // floats per plane
+  int nx, ny, lrow, hw;
+  float uscale, vscale;
+  float shift[3];
+  float rot[9];
+  float guardu, guardv;  // wavelengths, CPU arithmetic
+  float maxBL2, minBL2, bmTaper;
+  int doBeam, doTape;
+  int ntu, ntv;
+  long long listBase;  // first list of this facet (plane 0, tile 0)
+};
+
+struct Common {
+  int nrparm, lenvis, nchan, nplane, convWidth, convNperCell;
+  const float *freq;    // FP32 fscale per channel
+  const float *sig1, *sig2, *sig3;
+  int hasSigma;
+  const int *pchan;     // channels of each plane [nplane][maxpc]
+  const int *pcount;    // [nplane]
+  const float *pfmin, *pfmax;  // plane frequency extremes
+  int maxpc;
+  const float *gcf;     // [708]
+};
+
+// ---------------- device: CPU-exact sample arithmetic ----------------
+
+// One lane of the SSE Cephes sincos the CPU evaluates (same constants and
+// operation order as the oracle's restatement; FP contraction off).
+__device__ __forceinline__ void cephes_sincos(float xin, float *s, float *c) {
+#pragma clang fp contract(off)
+  const float FOPI = 1.27323954473516f;
+  const float MDP1 = -0.78515625f, MDP2 = -2.4187564849853515625e-4f,
+              MDP3 = -3.77489497744594108e-8f;
+  const float S0 = -1.9515295891E-4f, S1 = 8.3321608736E-3f, S2 = -1.6666654611E-1f;
+  const float C0 = 2.443315711809948E-005f, C1 = -1.388731625493765E-003f,
+              C2 = 4.166664568298827E-002f;
+  unsigned xb = __float_as_uint(xin);
+  unsigned sign_sin = xb & 0x80000000u;
+  float x = __uint_as_float(xb & 0x7fffffffu);
+  float y = x * FOPI;
+  int j = (int)y;
+  j = (j + 1) & ~1;
+  y = (float)j;
+  unsigned swap_sin = ((unsigned)(j & 4)) << 29;
+  bool poly_mask = (j & 2) == 0;
+  float t1 = y * MDP1, t2 = y * MDP2, t3 = y * MDP3;
+  x = x + t1;
+  x = x + t2;
+  x = x + t3;
+  unsigned sign_cos = ((unsigned)(~(j - 2) & 4)) << 29;
+  sign_sin ^= swap_sin;
+  float z = x * x;
+  float yc = C0;
+  yc = yc * z; yc = yc + C1;
+  yc = yc * z; yc = yc + C2;
+  yc = yc * z; yc = yc * z;
+  float tmp = z * 0.5f;
+  yc = yc - tmp;
+  yc = yc + 1.0f;
+  float ys = S0;
+  ys = ys * z; ys = ys + S1;
+  ys = ys * z; ys = ys + S2;
+  ys = ys * z; ys = ys * x;
+  ys = ys + x;
+  float ysin2 = poly_mask ? ys : 0.0f;
+  float ysin1 = poly_mask ? 0.0f : yc;
+  float ys_r = ys - ysin2;
+  float yc_r = yc - ysin1;
+  *s = __uint_as_float(__float_as_uint(ysin1 + ysin2) ^ sign_sin);
+  *c = __uint_as_float(__float_as_uint(yc_r + ys_r) ^ sign_cos);
+}
+
+__device__ __forceinline__ int lround_obit(float in) {
+#pragma clang fp contract(off)
+  if (in >= 0.0f) return (int)((double)in + 0.5);
+  return (int)((double)in - 0.5);
+}
+
+struct Sample {
+  int iu, iv, itu, itv;  // first column, first row, GCF blocks
+  float vr, vi;          // weighted visibility; (0,0) means "not gridded"
+};
+
+// Rotated uvw of one row for a facet (float, left-to-right, no FMA).
+__device__ __forceinline__ void rotate_uvw(const FacetDev &f, const float *row, float &uu,
+                                           float &vv, float &ww) {
+#pragma clang fp contract(off)
+  const float u = row[0], v = row[1], w = row[2];
+  uu = u * f.rot[0] + v * f.rot[1] + w * f.rot[2];
+  vv = u * f.rot[3] + v * f.rot[4] + w * f.rot[5];
+  ww = u * f.rot[6] + v * f.rot[7] + w * f.rot[8];
+}
+
+// Cells of a channel at FP32 frequency factor freq (no validity test):
+// used by the binning pass to bound a track's footprint.
+__device__ __forceinline__ void cells_at(const FacetDev &f, float uu, float vv, float freq,
+                                         int &iu, int &ivrow) {
+#pragma clang fp contract(off)
+  const float phaseSign = (uu <= 0.0f) ? -1.0f : 1.0f;
+  const float ff = phaseSign * freq;
+  float u = uu * ff, v = vv * ff;
+  u = u * f.uscale;
+  v = v * f.vscale;
+  iu = (int)((double)u + 0.5);
+  ivrow = lround_obit(v) - f.hw + f.ny / 2;
+}
+
+// fast_rot + fast_prep_grid for one (row, channel) of one facet.
+__device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *row, float uu,
+                              float vv, float ww, int ich) {
+#pragma clang fp contract(off)
+  Sample s;
+  const float phaseSign = (uu <= 0.0f) ? -1.0f : 1.0f;
+  const float freqFact = phaseSign * cm.freq[ich];
+  float u = uu * freqFact, v = vv * freqFact, w = ww * freqFact;
+  const float *vis = row + cm.nrparm + ich * 3;
+  // fast_rot: phase shift (double phase, reduced, FP32 Cephes sincos)
+  float vr, vi;
+  if (f.doBeam) {
+    vr = 1.0f; vi = 0.0f;
+  } else {
+    const double twopi = 2 * 3.14159265358979323846;
+    const double itwopi = 1.0 / (2 * 3.14159265358979323846);
+    const double p0 = (double)u * (double)f.shift[0];
+    const double p1 = (double)v * (double)f.shift[1];
+    const double p2 = (double)w * (double)f.shift[2];
+    const double dphase = (p0 + p1) + p2;
+    const int iphase = (int)(dphase * itwopi);
+    const float ph = (float)(dphase - (double)iphase * twopi);
+    float sn, cs;
+    cephes_sincos(ph, &sn, &cs);
+    const float vvr = vis[0];
+    const float vvi = vis[1] * phaseSign;
+    vr = vvr * cs - vvi * sn;
+    vi = vvr * sn + vvi * cs;
+  }
+  // fast_prep_grid
+  float vw = vis[2];
+  bool want = (vw > 0.0f) && (u < f.guardu) && (fabsf(v) < f.guardv);
+  const float BL2 = u * u + v * v;
+  if ((f.maxBL2 > 0.0f) && want) want = f.maxBL2 > BL2;
+  if ((f.minBL2 > 0.0f) && want) want = f.minBL2 < BL2;
+  if (!want) {
+    s.iu = f.hw; s.iv = f.ny / 2; s.itu = 0; s.itv = 0; s.vr = 0.0f; s.vi = 0.0f;
+    return s;
+  }
+  if (f.doTape) {
+    if (cm.hasSigma && cm.sig1[ich] != 0.0f) {
+      const float ftemp = u * u * (cm.sig2[ich] + f.bmTaper) + v * v * (cm.sig1[ich] + f.bmTaper) +
+                          u * v * (cm.sig3[ich]);
+      vw *= expf(ftemp);
+    } else if (fabsf(f.bmTaper) > 0.0f) {
+      vw *= expf(f.bmTaper * BL2);
+    }
+  }
+  s.vr = vr * vw;
+  s.vi = vi * vw;
+  u = u * f.uscale;
+  v = v * f.vscale;
+  const int iu = (int)((double)u + 0.5);
+  const int iv = lround_obit(v);
+  const int N = cm.convNperCell;
+  const float du = (float)iu - u;
+  const float dv = (float)iv - v;
+  s.itu = N + lround_obit((float)((double)N * ((double)du - 0.5)));
+  s.itv = N + lround_obit((float)((double)N * ((double)dv - 0.5)));
+  if (s.itu < 0 || s.itu > N || s.itv < 0 || s.itv > N) {  // only for non-positive frequencies
+    s.iu = f.hw; s.iv = f.ny / 2; s.itu = 0; s.itv = 0; s.vr = 0.0f; s.vi = 0.0f;
+    return s;
+  }
+  s.iu = iu;
+  s.iv = iv - f.hw + f.ny / 2;
+  return s;
+}
+
+// ---------------- binning ----------------
+
+// Clear two int arrays with a kernel (kept out of memset nodes so the
+// sequence stays capturable in a hipGraph).
+__global__ void k_zero2(int *a, int *b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) { a[i] = 0; b[i] = 0; }
+}
+
+// mode 0: count tracks per tile; mode 1: write row indices into lists.
+__global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, int nf,
+                      Common cm, int *counts, int *cursor, const int *offsets, int *lists,
+                      int mode) {
+  const long long total = nrow * (long long)cm.nplane * nf;
+  for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
+       it += (long long)gridDim.x * blockDim.x) {
+    const long long r = it % nrow;
+    const long long fp = it / nrow;
+    const int p = (int)(fp % cm.nplane);
+    const int fi = (int)(fp / cm.nplane);
+    if (cm.pcount[p] <= 0) continue;
+    const FacetDev &f = facets[fi];
+    const float *row = vis + r * (long long)cm.lenvis;
+    float uu, vv, ww;
+    rotate_uvw(f, row, uu, vv, ww);
+    int iu0, iv0, iu1, iv1;
+    cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
+    cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
+    const int ncol = f.lrow / 2;
+    int cA = min(iu0, iu1), cB = max(iu0, iu1) + cm.convWidth - 1;
+    int rA = min(iv0, iv1), rB = max(iv0, iv1) + cm.convWidth - 1;
+    cA = max(cA, 0); rA = max(rA, 0);
+    cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
+    if (cA > cB || rA > rB) continue;
+    const int tu0 = cA / TU, tu1 = cB / TU, tv0 = rA / TV, tv1 = rB / TV;
+    const long long base = f.listBase + (long long)p * f.ntu * f.ntv;
+    for (int tv = tv0; tv <= tv1; ++tv)
+      for (int tu = tu0; tu <= tu1; ++tu) {
+        const long long L = base + (long long)tv * f.ntu + tu;
+        if (mode == 0) {
+          atomicAdd(&counts[L], 1);
+        } else {
+          const int pos = atomicAdd(&cursor[L], 1);
+          lists[offsets[L] + pos] = (int)r;
+        }
+      }
+  }
+}
+
+// ---------------- tile gridding ----------------
+
+// One workgroup per (facet, plane, tile) list.  LDS holds the tile as
+// separate real and imaginary planes (TV x TU floats each) and the GCF.
+__global__ __launch_bounds__(kThreads) void k_grid_tiles(const float *vis, const FacetDev *facets,
+                                                         const int *listFacet, long long nlist,
+                                                         Common cm, const int *counts,
+                                                         const int *offsets, const int *lists) {
+  __shared__ float s_re[TV * TU];
+  __shared__ float s_im[TV * TU];
+  __shared__ float s_gcf[kGcf];
+  for (int i = threadIdx.x; i < kGcf; i += kThreads) s_gcf[i] = cm.gcf[i];
+  for (long long L = blockIdx.x; L < nlist; L += gridDim.x) {
+    const int n = counts[L];
+    if (n == 0) continue;
+    const int fi = listFacet[L];
+    const FacetDev &f = facets[fi];
+    const long long rel = L - f.listBase;
+    const int per = f.ntu * f.ntv;
+    const int p = (int)(rel / per);
+    const int t = (int)(rel - (long long)p * per);
+    const int tv = t / f.ntu, tu = t - tv * f.ntu;
+    const int col0 = tu * TU, row0 = tv * TV;
+    __syncthreads();
+    for (int i = threadIdx.x; i < TV * TU; i += kThreads) { s_re[i] = 0.0f; s_im[i] = 0.0f; }
+    __syncthreads();
+    const int *lst = lists + offsets[L];
+    const int npc = cm.pcount[p];
+    const int *pch = cm.pchan + (size_t)p * cm.maxpc;
+    const int W = cm.convWidth;
+    for (int e = threadIdx.x; e < n; e += kThreads) {
+      const long long r = lst[e];
+      const float *row = vis + r * (long long)cm.lenvis;
+      float uu, vv, ww;
+      rotate_uvw(f, row, uu, vv, ww);
+      for (int k = 0; k < npc; ++k) {
+        const Sample s = prep_sample(f, cm, row, uu, vv, ww, pch[k]);
+        if (s.vr == 0.0f && s.vi == 0.0f) continue;
+        const int c0 = s.iu - col0, r0 = s.iv - row0;
+        if (c0 + W <= 0 || c0 >= TU || r0 + W <= 0 || r0 >= TV) continue;
+        const float *cu = s_gcf + W * s.itu;
+        const float *cv = s_gcf + W * s.itv;
+        for (int jv = 0; jv < W; ++jv) {
+          const int rr = r0 + jv;
+          if (rr < 0 || rr >= TV) continue;
+          const float cvv = cv[jv];
+          if (cvv == 0.0f) continue;
+          for (int ju = 0; ju < W; ++ju) {
+            const int cc = c0 + ju;
+            if (cc < 0 || cc >= TU) continue;
+            const float cfn = cu[ju] * cvv;
+            atomicAdd(&s_re[rr * TU + cc], s.vr * cfn);
+            atomicAdd(&s_im[rr * TU + cc], s.vi * cfn);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // add the tile into the grid (each cell belongs to this workgroup only)
+    float *g = f.grid + (size_t)p * f.sizeGrid;
+    const int ncol = f.lrow / 2;
+    for (int i = threadIdx.x; i < TV * TU; i += kThreads) {
+      const int rr = i / TU, cc = i - rr * TU;
+      const int gr = row0 + rr, gc = col0 + cc;
+      if (gr >= f.ny || gc >= ncol) continue;
+      const float re = s_re[i], im = s_im[i];
+      if (re == 0.0f && im == 0.0f) continue;
+      float *q = g + (size_t)gr * f.lrow + 2 * gc;
+      q[0] += re;
+      q[1] += im;
+    }
+  }
+}
+
+// ThreadFlip (src/ObitThreadGrid.c:998-1045) as a parallel update: thread
+// (facet plane, vrow in 1..ny/2, iu in 0..hw).  Each thread reads the
+// cells at hw-iu (never written) and hw+iu (written only by itself).
+__global__ void k_flip(const FacetDev *facets, int nf, int nplane, int hwmax) {
+  const int fi = blockIdx.z / nplane, p = blockIdx.z - (blockIdx.z / nplane) * nplane;
+  if (fi >= nf) return;
+  const FacetDev &f = facets[fi];
+  const int ny2 = f.ny / 2;
+  const int hw = f.hw;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vrow = 1 + idx / (hwmax + 1);
+  const int iu = idx - (idx / (hwmax + 1)) * (hwmax + 1);
+  if (vrow > ny2 || iu > hw) return;
+  float *g = f.grid + (size_t)p * f.sizeGrid;
+  const int vc = f.ny - vrow;
+  float *gxo = g + (size_t)vrow * f.lrow + 2 * (hw + iu);
+  float *gxi = g + (size_t)vrow * f.lrow + 2 * (hw - iu);
+  float *gco = g + (size_t)vc * f.lrow + 2 * (hw + iu);
+  float *gci = g + (size_t)vc * f.lrow + 2 * (hw - iu);
+  const float cji0 = gci[0], cji1 = gci[1], cjo0 = gco[0], cjo1 = gco[1];
+  const float xxi0 = gxi[0], xxi1 = gxi[1], xxo0 = gxo[0], xxo1 = gxo[1];
+  gxo[0] = xxo0 + cji0;
+  gxo[1] = xxo1 - cji1;
+  gco[0] = cjo0 + xxi0;
+  gco[1] = cjo1 - xxi1;
+}
+
+// Trim hw columns and swap v halves: out[p][y][x] = grid[p][(y+ny/2)%ny][hw+x].
+__global__ void k_trim_centre(const FacetDev *facets, int fi, int nplane, float *out) {
+  const FacetDev &f = facets[fi];
+  const int nxo = f.nx / 2 + 1;
+  const long long total = (long long)nplane * f.ny * nxo;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nxo);
+    const long long yp = i / nxo;
+    const int y = (int)(yp % f.ny);
+    const int p = (int)(yp / f.ny);
+    const int ys = (y + f.ny / 2) % f.ny;
+    const float *src = f.grid + (size_t)p * f.sizeGrid + (size_t)ys * f.lrow + 2 * (f.hw + x);
+    out[2 * i] = src[0];
+    out[2 * i + 1] = src[1];
+  }
+}
+
+// Per-sample outputs of prep_sample for the parity tests.
+__global__ void k_prep_debug(const float *vis, long long nrow, const FacetDev *facets, int fi,
+                             Common cm, int *iu, int *iv, int *itu, int *itv, float *vout) {
+  const long long total = nrow * cm.nchan;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long r = i / cm.nchan;
+    const int c = (int)(i - r * cm.nchan);
+    const FacetDev &f = facets[fi];
+    const float *row = vis + r * (long long)cm.lenvis;
+    float uu, vv, ww;
+    rotate_uvw(f, row, uu, vv, ww);
+    const Sample s = prep_sample(f, cm, row, uu, vv, ww, c);
+    iu[i] = s.iu; iv[i] = s.iv; itu[i] = s.itu; itv[i] = s.itv;
+    vout[2 * i] = s.vr; vout[2 * i + 1] = s.vi;
+  }
+}
+
+// ---------------- host state ----------------
+
+struct GpuState {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nCU = 256;
+  std::vector<int> facets;  // facet indices on this GPU
+  FacetDev *d_facets = nullptr;
+  std::vector<FacetDev> h_facets;
+  int *d_listFacet = nullptr;
+  long long nlist = 0;
+  int *d_counts = nullptr, *d_cursor = nullptr, *d_offsets = nullptr, *d_lists = nullptr;
+  long long listCap = 0;
+  void *d_scanTmp = nullptr;
+  size_t scanTmpCap = 0;
+  // tables
+  float *d_gcf = nullptr, *d_freq = nullptr, *d_sig = nullptr, *d_pf = nullptr;
+  int *d_pchan = nullptr, *d_pcount = nullptr;
+  Common cm{};
+  bool ready = false;
+  // staging for host buffers
+  float *d_vis = nullptr;
+  size_t visCap = 0;
+};
+
+struct GridCtx {
+  std::vector<GpuState *> gpus;
+};
+
+std::mutex g_mu;
+std::unordered_map<const CUDAGridInfo *, GridCtx *> g_ctx;
+
+GridCtx *get_ctx(const CUDAGridInfo *in, bool create) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_ctx.find(in);
+  if (it != g_ctx.end()) return it->second;
+  if (!create) return nullptr;
+  GridCtx *c = new GridCtx();
+  g_ctx[in] = c;
+  return c;
+}
+
+GpuState *get_gpu(CUDAGridInfo *in, long iGPU, bool create) {
+  GridCtx *c = get_ctx(in, create);
+  if (!c) return nullptr;
+  if ((long)c->gpus.size() <= iGPU) {
+    if (!create) return nullptr;
+    c->gpus.resize(iGPU + 1, nullptr);
+  }
+  if (!c->gpus[iGPU] && create) c->gpus[iGPU] = new GpuState();
+  return c->gpus[iGPU];
+}
+
+int device_of(const CUDAGridInfo *in, long iGPU) {
+  if (in->h_gridInfo && in->h_gridInfo->GPU_device_no) return in->h_gridInfo->GPU_device_no[iGPU];
+  if (in->cuda_device) return (int)in->cuda_device[iGPU];
+  return (int)iGPU;
+}
+
+// Build the CPU gridding-order table from the tabulated GCF the host
+// passes (h_convfn = ObitUVGrid convfn, 100 points per cell):
+// gcf[7*it + c] = convfn[c*100 + it], it = 0..100 (src/ObitUVGrid.c:1398-1409).
+void build_gcf(const float *convfn, int W, int N, std::vector<float> &gcf) {
+  gcf.assign(kGcf, 0.0f);
+  for (int it = 0; it <= N; ++it)
+    for (int c = 0; c < W; ++c) {
+      const int src = c * N + it;
+      const int dst = W * it + c;
+      if (dst < kGcf && src < W * N + 1) gcf[dst] = convfn[src];
+    }
+}
+
+// Upload tables and facet parameters from the host descriptors.
+int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
+  GridInfo *gi = in->h_gridInfo;
+  if (!gi) return fail(kState, "h_gridInfo not allocated");
+  OBIT_HIP_TRY(hipSetDevice(gs->device));
+  const int nchan = gi->nchan, nplane = gi->nplane;
+  const int W = gi->convWidth, N = gi->convNperCell;
+  if (W != 7 || N != 100) return fail(kInvalid, "GCF must be 7 cells x 100 points (got %d x %d)", W, N);
+  if (nchan <= 0 || nplane <= 0) return fail(kInvalid, "nchan/nplane must be positive");
+  if (gi->lenvis < gi->nrparm + 3 * nchan || gi->nrparm < 3)
+    return fail(kInvalid, "rows of %d floats cannot hold %d random parameters and %d channels",
+                gi->lenvis, gi->nrparm, nchan);
+  // facets on this GPU
+  gs->facets.clear();
+  for (long i = 0; i < in->nfacet; ++i)
+    if (!in->FacetGPU || in->FacetGPU[i] == iGPU) gs->facets.push_back((int)i);
+  const int nf = (int)gs->facets.size();
+  gs->h_facets.assign(nf, FacetDev{});
+  long long listBase = 0;
+  std::vector<int> listFacet;
+  for (int k = 0; k < nf; ++k) {
+    const FacetInfo *fi = in->h_facetInfo[gs->facets[k]];
+    FacetDev &f = gs->h_facets[k];
+    f.grid = fi->d_grid;
+    f.sizeGrid = fi->sizeGrid;
+    f.nx = fi->nx; f.ny = fi->ny; f.hw = W / 2;
+    f.lrow = 2 * (1 + fi->nx / 2 + f.hw);
+    if (fi->sizeGrid < f.lrow * fi->ny) return fail(kInvalid, "facet %d: sizeGrid too small", gs->facets[k]);
+    f.uscale = fi->uscale; f.vscale = fi->vscale;
+    for (int j = 0; j < 3; ++j) f.shift[j] = fi->shift[j];
+    for (int j = 0; j < 9; ++j) f.rot[j] = fi->rotUV[j];
+    // guard band in wavelengths as src/ObitThreadGrid.c:1290-1291 forms it
+    // from guardband*nx (src/ObitThreadGrid.c:328-329), in float
+    const volatile float gcu = gi->guardu * (float)fi->nx;
+    const volatile float gcv = gi->guardv * (float)fi->ny;
+    const volatile float gqu = gcu / fi->uscale;
+    const volatile float gqv = gcv / fi->vscale;
+    f.guardu = std::fabs((float)gqu);
+    f.guardv = std::fabs((float)gqv);
+    const volatile float mx = fi->maxBL * fi->maxBL, mn = fi->minBL * fi->minBL;
+    f.maxBL2 = mx; f.minBL2 = mn;
+    f.bmTaper = fi->bmTaper;
+    f.doBeam = fi->doBeam;
+    f.doTape = (std::fabs(fi->bmTaper) > 0.0f) || (gi->h_sigma1 && gi->h_sigma1[0] != 0.0f);
+    f.ntu = (f.lrow / 2 + TU - 1) / TU;
+    f.ntv = (f.ny + TV - 1) / TV;
+    f.listBase = listBase;
+    const long long nl = (long long)nplane * f.ntu * f.ntv;
+    listFacet.insert(listFacet.end(), (size_t)nl, k);
+    listBase += nl;
+  }
+  gs->nlist = listBase;
+  // tables
+  std::vector<float> gcf;
+  build_gcf(gi->h_convfn, W, N, gcf);
+  std::vector<std::vector<int>> pch(nplane);
+  std::vector<float> pf(2 * nplane, 0.0f);
+  for (int c = 0; c < nchan; ++c) {
+    const int p = gi->h_freqPlane[c];
+    if (p >= 0 && p < nplane) pch[p].push_back(c);
+  }
+  int maxpc = 1;
+  for (int p = 0; p < nplane; ++p) maxpc = std::max(maxpc, (int)pch[p].size());
+  std::vector<int> hpch((size_t)nplane * maxpc, 0), hpcount(nplane, 0);
+  for (int p = 0; p < nplane; ++p) {
+    hpcount[p] = (int)pch[p].size();
+    float lo = 0.0f, hi = 0.0f;
+    for (size_t k = 0; k < pch[p].size(); ++k) {
+      hpch[(size_t)p * maxpc + k] = pch[p][k];
+      const float fr = gi->h_freqArr[pch[p][k]];
+      if (k == 0 || fr < lo) lo = fr;
+      if (k == 0 || fr > hi) hi = fr;
+    }
+    pf[p] = lo; pf[nplane + p] = hi;
+  }
+  bool hasSigma = false;
+  if (gi->h_sigma1)
+    for (int c = 0; c < nchan; ++c) hasSigma = hasSigma || gi->h_sigma1[c] != 0.0f;
+  // (re)allocate device tables
+  (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
+  (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
+  OBIT_HIP_TRY(hipMalloc(&gs->d_gcf, sizeof(float) * kGcf));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_freq, sizeof(float) * nchan));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_sig, sizeof(float) * 3 * nchan));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_pf, sizeof(float) * 2 * nplane));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_pchan, sizeof(int) * hpch.size()));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_pcount, sizeof(int) * nplane));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_facets, sizeof(FacetDev) * std::max(nf, 1)));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_listFacet, sizeof(int) * std::max<long long>(gs->nlist, 1)));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_gcf, gcf.data(), sizeof(float) * kGcf, hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_freq, gi->h_freqArr, sizeof(float) * nchan, hipMemcpyHostToDevice));
+  std::vector<float> sig(3 * (size_t)nchan, 0.0f);
+  if (gi->h_sigma1) std::memcpy(sig.data(), gi->h_sigma1, sizeof(float) * nchan);
+  if (gi->h_sigma2) std::memcpy(sig.data() + nchan, gi->h_sigma2, sizeof(float) * nchan);
+  if (gi->h_sigma3) std::memcpy(sig.data() + 2 * nchan, gi->h_sigma3, sizeof(float) * nchan);
+  OBIT_HIP_TRY(hipMemcpy(gs->d_sig, sig.data(), sizeof(float) * 3 * nchan, hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_pf, pf.data(), sizeof(float) * 2 * nplane, hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_pchan, hpch.data(), sizeof(int) * hpch.size(), hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_pcount, hpcount.data(), sizeof(int) * nplane, hipMemcpyHostToDevice));
+  if (nf) OBIT_HIP_TRY(hipMemcpy(gs->d_facets, gs->h_facets.data(), sizeof(FacetDev) * nf, hipMemcpyHostToDevice));
+  if (gs->nlist)
+    OBIT_HIP_TRY(hipMemcpy(gs->d_listFacet, listFacet.data(), sizeof(int) * gs->nlist, hipMemcpyHostToDevice));
+  Common &cm = gs->cm;
+  cm.nrparm = gi->nrparm; cm.lenvis = gi->lenvis; cm.nchan = nchan; cm.nplane = nplane;
+  cm.convWidth = W; cm.convNperCell = N;
+  cm.freq = gs->d_freq;
+  cm.sig1 = gs->d_sig; cm.sig2 = gs->d_sig + nchan; cm.sig3 = gs->d_sig + 2 * nchan;
+  cm.hasSigma = hasSigma ? 1 : 0;
+  cm.pchan = gs->d_pchan; cm.pcount = gs->d_pcount;
+  cm.pfmin = gs->d_pf; cm.pfmax = gs->d_pf + nplane;
+  cm.maxpc = maxpc;
+  cm.gcf = gs->d_gcf;
+  // list buffers sized for the lists
+  (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets);
+  OBIT_HIP_TRY(hipMalloc(&gs->d_counts, sizeof(int) * (gs->nlist + 1)));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_cursor, sizeof(int) * (gs->nlist + 1)));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_offsets, sizeof(int) * (gs->nlist + 1)));
+  size_t tmp = 0;
+  OBIT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, gs->d_counts, gs->d_offsets,
+                                                (int)(gs->nlist + 1), gs->stream));
+  if (tmp > gs->scanTmpCap) {
+    (void)hipFree(gs->d_scanTmp);
+    OBIT_HIP_TRY(hipMalloc(&gs->d_scanTmp, tmp));
+    gs->scanTmpCap = tmp;
+  }
+  gs->ready = true;
+  return kOK;
+}
+
+// Bin and grid nrow rows at a device pointer into every facet on gs.
+int grid_device(GpuState *gs, const float *vis, long long nrow) {
+  if (!gs->ready) return fail(kState, "gridder not initialised");
+  if (nrow <= 0 || gs->nlist == 0) return kOK;
+  const int nf = (int)gs->facets.size();
+  const long long nl = gs->nlist;
+  if ((long long)nrow * gs->cm.nplane * nf > (1LL << 40)) return fail(kInvalid, "too many tracks");
+  {
+    const int nbz = (int)std::min<long long>((nl + 1 + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_zero2, dim3(nbz), dim3(256), 0, gs->stream, gs->d_counts, gs->d_cursor, nl + 1);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  const long long ntr = nrow * (long long)gs->cm.nplane * nf;
+  const int nbB = (int)std::min<long long>((ntr + 255) / 256, (long long)gs->nCU * 32);
+  hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
+                     gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 0);
+  OBIT_HIP_TRY(hipGetLastError());
+  size_t tmp = gs->scanTmpCap;
+  OBIT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(gs->d_scanTmp, tmp, gs->d_counts, gs->d_offsets,
+                                                (int)(nl + 1), gs->stream));
+  int total = 0;
+  OBIT_HIP_TRY(hipMemcpyAsync(&total, gs->d_offsets + nl, sizeof(int), hipMemcpyDeviceToHost, gs->stream));
+  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+  if (total < 0) return fail(kInvalid, "list size overflow");
+  if ((long long)total > gs->listCap) {
+    (void)hipFree(gs->d_lists);
+    gs->d_lists = nullptr;
+    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int) * (size_t)std::max(total, 1));
+    if (e != hipSuccess) return fail(kNoMemory, "track lists (%d): %s", total, hipGetErrorString(e));
+    gs->listCap = total;
+  }
+  hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
+                     gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
+  OBIT_HIP_TRY(hipGetLastError());
+  const long long nbG = std::min<long long>(nl, (long long)gs->nCU * 8);
+  hipLaunchKernelGGL(k_grid_tiles, dim3((unsigned)nbG), dim3(kThreads), 0, gs->stream, vis,
+                     gs->d_facets, gs->d_listFacet, nl, gs->cm, gs->d_counts, gs->d_offsets,
+                     gs->d_lists);
+  OBIT_HIP_TRY(hipGetLastError());
+  return kOK;
+}
+
+}  // namespace grid
+}  // namespace obit
+
+using obit::grid::GpuState;
+
+static void grid_abort(const char *where) {
+  std::fprintf(stderr, "obit_hip: %s: %s\n", where, ObitHIPLastError());
+  std::abort();
+}
+
+extern "C" {
+
+void ObitCUDAGridAlloc(CUDAGridInfo *cudaInfo, long nfacet, long nchan, long nif, long nVisPIO,
+                       long lenvis, long nplane, long iGPU, int doBuff) {
+  if (!cudaInfo || nfacet <= 0 || iGPU < 0 || iGPU >= (cudaInfo->nGPU > 0 ? cudaInfo->nGPU : 1)) {
+    std::fprintf(stderr, "obit_hip: ObitCUDAGridAlloc: bad arguments\n");
+    std::abort();
+  }
+  const long nGPU = cudaInfo->nGPU > 0 ? cudaInfo->nGPU : 1;
+  cudaInfo->nVisPIO = nVisPIO;
+  cudaInfo->nfacet = nfacet;
+  if (!cudaInfo->h_gridInfo) {
+    cudaInfo->h_gridInfo = (GridInfo *)ObitCUDAUtilAllocHost(sizeof(GridInfo));
+    std::memset(cudaInfo->h_gridInfo, 0, sizeof(GridInfo));
+    cudaInfo->h_gridInfo->nStream = GRID_STREAM_COUNT;
+    cudaInfo->h_gridInfo->stream = (int **)std::calloc(nGPU, sizeof(int *));
+    cudaInfo->h_gridInfo->cycleDone = (int **)std::calloc(nGPU, sizeof(int *));
+  }
+  GridInfo *gi = cudaInfo->h_gridInfo;
+  gi->nfacet = (int)nfacet;
+  gi->nchan = (int)(nchan * nif);
+  gi->nif = (int)nif;
+  gi->nvis = (int)nVisPIO;
+  gi->lenvis = (int)lenvis;
+  gi->nplane = (int)nplane;
+  gi->nGPU = (int)nGPU;
+  const size_t nci = (size_t)(nchan * nif);
+  // the reference sizes the plane table with sizeof(long) per entry
+  if (!gi->h_freqPlane) { gi->h_freqPlane = (int *)ObitCUDAUtilAllocHost((int)(nci * sizeof(long))); std::memset(gi->h_freqPlane, 0, nci * sizeof(long)); }
+  if (!gi->h_sigma1) {
+    gi->h_sigma1 = ObitCUDAUtilAllocHost((int)(nci * sizeof(float)));
+    gi->h_sigma2 = ObitCUDAUtilAllocHost((int)(nci * sizeof(float)));
+    gi->h_sigma3 = ObitCUDAUtilAllocHost((int)(nci * sizeof(float)));
+    std::memset(gi->h_sigma1, 0, nci * sizeof(float));
+    std::memset(gi->h_sigma2, 0, nci * sizeof(float));
+    std::memset(gi->h_sigma3, 0, nci * sizeof(float));
+  }
+  if (!gi->h_freqArr) { gi->h_freqArr = ObitCUDAUtilAllocHost((int)(nci * sizeof(float))); std::memset(gi->h_freqArr, 0, nci * sizeof(float)); }
+  if (!gi->h_convfn) { gi->h_convfn = ObitCUDAUtilAllocHost(2000 * sizeof(float)); std::memset(gi->h_convfn, 0, 2000 * sizeof(float)); }
+  if (doBuff && !gi->h_data_in) {
+    const size_t b = (size_t)nVisPIO * lenvis * sizeof(float);
+    void *p = nullptr;
+    OBIT_HIP_CHECK(hipHostMalloc(&p, b ? b : 4, hipHostMallocDefault));
+    gi->h_data_in = (float *)p;
+    cudaInfo->h_data_in = gi->h_data_in;
+  }
+  if (!cudaInfo->h_facetInfo) {
+    cudaInfo->h_facetInfo = (FacetInfo **)ObitCUDAUtilAllocHost((int)(nfacet * sizeof(FacetInfo *)));
+    for (long i = 0; i < nfacet; ++i) {
+      cudaInfo->h_facetInfo[i] = (FacetInfo *)ObitCUDAUtilAllocHost(sizeof(FacetInfo));
+      std::memset(cudaInfo->h_facetInfo[i], 0, sizeof(FacetInfo));
+      cudaInfo->h_facetInfo[i]->GPU_num = (int)iGPU;
+      cudaInfo->h_facetInfo[i]->rotUV[0] = 1.0f;
+      cudaInfo->h_facetInfo[i]->rotUV[4] = 1.0f;
+      cudaInfo->h_facetInfo[i]->rotUV[8] = 1.0f;
+    }
+  }
+  GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, true);
+  gs->device = obit::grid::device_of(cudaInfo, iGPU);
+  OBIT_HIP_CHECK(hipSetDevice(gs->device));
+  if (!gs->stream) OBIT_HIP_CHECK(hipStreamCreateWithFlags(&gs->stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  OBIT_HIP_CHECK(hipGetDeviceProperties(&prop, gs->device));
+  gs->nCU = prop.multiProcessorCount;
+  if (!gi->stream[iGPU]) gi->stream[iGPU] = (int *)gs->stream;
+}
+
+void ObitCUDAGridInit(CUDAGridInfo *cudaInfo, long iGPU) {
+  GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, false);
+  if (!gs) { std::fprintf(stderr, "obit_hip: ObitCUDAGridInit before Alloc\n"); std::abort(); }
+  gs->device = obit::grid::device_of(cudaInfo, iGPU);
+  OBIT_HIP_CHECK(hipSetDevice(gs->device));
+  const int nplane = cudaInfo->h_gridInfo->nplane;
+  for (long i = 0; i < cudaInfo->nfacet; ++i) {
+    if (cudaInfo->FacetGPU && cudaInfo->FacetGPU[i] != iGPU) continue;
+    FacetInfo *fi = cudaInfo->h_facetInfo[i];
+    const size_t bytes = (size_t)fi->sizeGrid * nplane * sizeof(float);
+    if (!fi->d_grid) OBIT_HIP_CHECK(hipMalloc(&fi->d_grid, bytes));
+    if (!fi->h_grid) {
+      void *p = nullptr;
+      OBIT_HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+      fi->h_grid = (float *)p;
+      std::memset(fi->h_grid, 0, bytes);
+    }
+    OBIT_HIP_CHECK(hipMemsetAsync(fi->d_grid, 0, bytes, gs->stream));
+  }
+  OBIT_HIP_CHECK(hipStreamSynchronize(gs->stream));
+  if (obit::grid::refresh(cudaInfo, iGPU, gs)) grid_abort("ObitCUDAGridInit");
+}
+
+void ObitCUDAGridGrid(CUDAGridInfo *cudaInfo, long iGPU, int init) {
+  GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, false);
+  if (!gs) { std::fprintf(stderr, "obit_hip: ObitCUDAGridGrid before Alloc\n"); std::abort(); }
+  OBIT_HIP_CHECK(hipSetDevice(gs->device));
+  if (init || !gs->ready)
+    if (obit::grid::refresh(cudaInfo, iGPU, gs)) grid_abort("ObitCUDAGridGrid");
+  GridInfo *gi = cudaInfo->h_gridInfo;
+  const long long nvis = gi->nvis;
+  if (nvis <= 0) return;
+  const size_t bytes = (size_t)nvis * gi->lenvis * sizeof(float);
+  if (bytes > gs->visCap) {
+    (void)hipFree(gs->d_vis);
+    OBIT_HIP_CHECK(hipMalloc(&gs->d_vis, bytes));
+    gs->visCap = bytes;
+  }
+  OBIT_HIP_CHECK(hipMemcpyAsync(gs->d_vis, gi->h_data_in, bytes, hipMemcpyHostToDevice, gs->stream));
+  if (obit::grid::grid_device(gs, gs->d_vis, nvis)) grid_abort("ObitCUDAGridGrid");
+  OBIT_HIP_CHECK(hipStreamSynchronize(gs->stream));
+}
+
+void ObitCUDAGridFlip(CUDAGridInfo *cudaInfo, long iGPU) {
+  GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, false);
+  if (!gs || !gs->ready) { std::fprintf(stderr, "obit_hip: ObitCUDAGridFlip before Init\n"); std::abort(); }
+  OBIT_HIP_CHECK(hipSetDevice(gs->device));
+  const int nf = (int)gs->facets.size();
+  const int nplane = cudaInfo->h_gridInfo->nplane;
+  int maxny = 0, hw = gs->cm.convWidth / 2;
+  for (auto &f : gs->h_facets) maxny = std::max(maxny, f.ny);
+  const long long n = (long long)(maxny / 2) * (hw + 1);
+  if (nf && n > 0) {
+    dim3 grid((unsigned)((n + 255) / 256), 1, (unsigned)(nf * nplane));
+    hipLaunchKernelGGL(obit::grid::k_flip, grid, dim3(256), 0, gs->stream, gs->d_facets, nf, nplane, hw);
+    OBIT_HIP_CHECK(hipGetLastError());
+  }
+  OBIT_HIP_CHECK(hipStreamSynchronize(gs->stream));
+}
+
+void ObitCUDAGrid2CPU(CUDAGridInfo *cudaInfo, long ifacet) {
+  FacetInfo *fi = cudaInfo->h_facetInfo[ifacet];
+  const long iGPU = cudaInfo->FacetGPU ? cudaInfo->FacetGPU[ifacet] : 0;
+  GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, false);
+  if (gs) OBIT_HIP_CHECK(hipSetDevice(gs->device));
+  const size_t bytes = (size_t)fi->sizeGrid * cudaInfo->h_gridInfo->nplane * sizeof(float);
+  OBIT_HIP_CHECK(hipMemcpy(fi->h_grid, fi->d_grid, bytes, hipMemcpyDeviceToHost));
+}
+
+void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
+  GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, false);
+  GridInfo *gi = cudaInfo->h_gridInfo;
+  const int nGPU = gi ? gi->nGPU : 1;
+  const bool last = (iGPU == nGPU - 1);
+  if (gs) {
+    (void)hipSetDevice(gs->device);
+    (void)hipStreamSynchronize(gs->stream);
+    for (long i = 0; i < cudaInfo->nfacet; ++i) {
+      if (cudaInfo->FacetGPU && cudaInfo->FacetGPU[i] != iGPU) continue;
+      FacetInfo *fi = cudaInfo->h_facetInfo ? cudaInfo->h_facetInfo[i] : nullptr;
+      if (fi && fi->d_grid) { (void)hipFree(fi->d_grid); fi->d_grid = nullptr; }
+    }
+    (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
+    (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
+    (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
+    (void)hipFree(gs->d_scanTmp); (void)hipFree(gs->d_vis);
+    if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
+    (void)hipStreamDestroy(gs->stream);
+    obit::grid::GridCtx *c = obit::grid::get_ctx(cudaInfo, false);
+    if (c) c->gpus[iGPU] = nullptr;
+    delete gs;
+  }
+  if (last && gi) {
+    if (gi->h_data_in) { (void)hipHostFree(gi->h_data_in); gi->h_data_in = nullptr; cudaInfo->h_data_in = nullptr; }
+    ObitCUDAUtilFreeHost(gi->h_freqArr);
+    ObitCUDAUtilFreeHost((float *)gi->h_freqPlane);
+    ObitCUDAUtilFreeHost(gi->h_sigma1);
+    ObitCUDAUtilFreeHost(gi->h_sigma2);
+    ObitCUDAUtilFreeHost(gi->h_sigma3);
+    ObitCUDAUtilFreeHost(gi->h_convfn);
+    if (gi->GPU_device_no) ObitCUDAUtilFreeHost((float *)gi->GPU_device_no);
+    std::free(gi->stream);
+    std::free(gi->cycleDone);
+    if (cudaInfo->h_facetInfo) {
+      for (long i = 0; i < cudaInfo->nfacet; ++i) {
+        FacetInfo *fi = cudaInfo->h_facetInfo[i];
+        if (!fi) continue;
+        if (fi->h_grid) (void)hipHostFree(fi->h_grid);
+        ObitCUDAUtilFreeHost((float *)fi);
+      }
+      ObitCUDAUtilFreeHost((float *)cudaInfo->h_facetInfo);
+      cudaInfo->h_facetInfo = nullptr;
+    }
+    ObitCUDAUtilFreeHost((float *)gi);
+    cudaInfo->h_gridInfo = nullptr;
+    std::lock_guard<std::mutex> lk(obit::grid::g_mu);
+    auto it = obit::grid::g_ctx.find(cudaInfo);
+    if (it != obit::grid::g_ctx.end()) { delete it->second; obit::grid::g_ctx.erase(it); }
+  }
+}
+
+// ---- extensions ----
+
+int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis) {
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  if (!gs) return obit::fail(obit::kState, "gridder not allocated on GPU index %ld", iGPU);
+  OBIT_HIP_TRY(hipSetDevice(gs->device));
+  if (!gs->ready) {
+    int rc = obit::grid::refresh(in, iGPU, gs);
+    if (rc) return rc;
+  }
+  return obit::grid::grid_device(gs, d_vis, nvis);
+}
+
+int ObitHIPGrid2Device(CUDAGridInfo *in, long ifacet, float *d_out) {
+  const long iGPU = in->FacetGPU ? in->FacetGPU[ifacet] : 0;
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  if (!gs || !gs->ready) return obit::fail(obit::kState, "gridder not initialised");
+  OBIT_HIP_TRY(hipSetDevice(gs->device));
+  int k = -1;
+  for (size_t i = 0; i < gs->facets.size(); ++i) if (gs->facets[i] == ifacet) k = (int)i;
+  if (k < 0) return obit::fail(obit::kInvalid, "facet %ld not on GPU index %ld", ifacet, iGPU);
+  const auto &f = gs->h_facets[k];
+  const long long total = (long long)in->h_gridInfo->nplane * f.ny * (f.nx / 2 + 1);
+  const int nb = (int)std::min<long long>((total + 255) / 256, (long long)gs->nCU * 16);
+  hipLaunchKernelGGL(obit::grid::k_trim_centre, dim3(std::max(nb, 1)), dim3(256), 0, gs->stream,
+                     gs->d_facets, k, in->h_gridInfo->nplane, d_out);
+  OBIT_HIP_TRY(hipGetLastError());
+  return obit::kOK;
+}
+
+int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out) {
+  FacetInfo *fi = in->h_facetInfo[ifacet];
+  const long long n = (long long)in->h_gridInfo->nplane * fi->ny * (fi->nx / 2 + 1) * 2;
+  float *d = nullptr;
+  OBIT_HIP_TRY(hipMalloc(&d, sizeof(float) * n));
+  int rc = ObitHIPGrid2Device(in, ifacet, d);
+  if (!rc) {
+    const long iGPU = in->FacetGPU ? in->FacetGPU[ifacet] : 0;
+    GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+    OBIT_HIP_TRY(hipMemcpy(h_out, d, sizeof(float) * n, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis, int *iu,
+                         int *iv, int *itu, int *itv, float *vis) {
+  const long iGPU = in->FacetGPU ? in->FacetGPU[ifacet] : 0;
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  if (!gs) return obit::fail(obit::kState, "gridder not allocated");
+  OBIT_HIP_TRY(hipSetDevice(gs->device));
+  if (!gs->ready) { int rc = obit::grid::refresh(in, iGPU, gs); if (rc) return rc; }
+  int k = -1;
+  for (size_t i = 0; i < gs->facets.size(); ++i) if (gs->facets[i] == ifacet) k = (int)i;
+  if (k < 0) return obit::fail(obit::kInvalid, "facet %ld not on GPU index %ld", ifacet, iGPU);
+  const int nch = gs->cm.nchan;
+  const long long ns = nvis * nch;
+  const size_t vb = (size_t)nvis * gs->cm.lenvis * sizeof(float);
+  float *dv = nullptr, *dvo = nullptr;
+  int *di = nullptr;
+  OBIT_HIP_TRY(hipMalloc(&dv, vb ? vb : 4));
+  OBIT_HIP_TRY(hipMalloc(&dvo, sizeof(float) * 2 * std::max<long long>(ns, 1)));
+  OBIT_HIP_TRY(hipMalloc(&di, sizeof(int) * 4 * std::max<long long>(ns, 1)));
+  OBIT_HIP_TRY(hipMemcpy(dv, h_vis, vb, hipMemcpyHostToDevice));
+  if (ns > 0) {
+    const int nb = (int)std::min<long long>((ns + 255) / 256, 4096);
+    hipLaunchKernelGGL(obit::grid::k_prep_debug, dim3(nb), dim3(256), 0, gs->stream, dv, nvis,
+                       gs->d_facets, k, gs->cm, di, di + ns, di + 2 * ns, di + 3 * ns, dvo);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+  OBIT_HIP_TRY(hipMemcpy(iu, di, sizeof(int) * ns, hipMemcpyDeviceToHost));
+  OBIT_HIP_TRY(hipMemcpy(iv, di + ns, sizeof(int) * ns, hipMemcpyDeviceToHost));
+  OBIT_HIP_TRY(hipMemcpy(itu, di + 2 * ns, sizeof(int) * ns, hipMemcpyDeviceToHost));
+  OBIT_HIP_TRY(hipMemcpy(itv, di + 3 * ns, sizeof(int) * ns, hipMemcpyDeviceToHost));
+  OBIT_HIP_TRY(hipMemcpy(vis, dvo, sizeof(float) * 2 * ns, hipMemcpyDeviceToHost));
+  (void)hipFree(dv); (void)hipFree(dvo); (void)hipFree(di);
+  return obit::kOK;
+}
+
+void *ObitHIPGridStream(CUDAGridInfo *in, long iGPU) {
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  return gs ? (void *)gs->stream : nullptr;
+}
+
+}  // extern "C"

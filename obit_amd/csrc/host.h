@@ -1,0 +1,100 @@
+/*
+ * host.h -- C host classes over the drop-in ABI.
+ *
+ * These play the role of Obit's ObitGPUSkyModel.c and ObitGPUGrid.c: they
+ * fill GPUInfo / CUDAGridInfo from a uv-data description and a sky model
+ * or facet list, then drive the ObitCUDA* entry points exactly as Obit's
+ * host classes do (reference src/ObitGPUSkyModel.c:207-579,
+ * src/ObitGPUGrid.c:255-721).  The Obit objects (ObitUV, ObitSkyModel,
+ * ObitUVGrid) are replaced by the plain structs below, which carry the
+ * fields those classes read.
+ */
+#ifndef OBIT_HIP_HOST_H
+#define OBIT_HIP_HOST_H
+#include "ObitCUDASkyModelInfoDef.h"
+#include "ObitCUDAGridInfoDef.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The ObitUVDesc fields the GPU host classes read (include/ObitUVDesc.h). */
+typedef struct {
+  int nvis, lrec, nrparm;
+  int ilocu, ilocv, ilocw, iloct, ilocb, ilocsu;
+  int nchan, nif, nstok;      /* inaxes of FREQ, IF (1 if absent), STOKES */
+  int freqBeforeIF;           /* 1: FREQ axis precedes IF (AIPS order) */
+  int hasIF;                  /* 1: the data have an IF axis */
+  int incs, incf, incif;      /* increments in floats (ObitUVDesc.c:380-392) */
+  int ncorr;
+  float crvalStokes;          /* crval of the STOKES axis (<0: correlations) */
+  double freq;                /* uv reference frequency (Hz) */
+  const double *freqArr;      /* [nchan*nif] channel frequencies (Hz) */
+  const float *fscale;        /* [nchan*nif] freqArr/freq as FP32 */
+} ObitHostUVDesc;
+
+/* The ObitSkyModel / ObitSkyModelMF fields the DFT host class reads. */
+typedef struct {
+  int modType;                /* ObitSkyModelCompType (include/ObitSkyModel.h:131) */
+  int nSpecTerm;
+  int doDivide, doReplace, doFlip;
+  float factor, stokFactor;
+  int startChannel, numberChannel, startIF, numberIF;  /* 1-relative selection */
+  int startPoln, numberPoln;  /* 1-relative Stokes selection */
+  int isMF;                   /* tabulated spectra (ObitSkyModelMF) */
+  int nSpec;
+  const double *specFreq;     /* [nSpec] coarse-bin frequencies (Hz) */
+  const int *specIndex;       /* [nchan*nif] coarse bin per channel */
+  double imgRefFreq;          /* image reference frequency (Hz), 0 if none */
+  int cuda_device;
+  int useFreq64;              /* pass FP64 frequencies to the library */
+} ObitHostSkyModel;
+
+/* The ObitUVGrid fields the gridding host class reads, per facet. */
+typedef struct {
+  int nx, ny;
+  float cellx, celly;         /* cell size (deg) */
+  float shift[3];             /* dxc, dyc, dzc */
+  float rotUV[9];             /* URot3D */
+  float blmax, blmin;
+  float bmTaperUV;
+  int doBeam;
+} ObitHostFacet;
+
+/* ---- DFT (ObitGPUSkyModel) ---- */
+GPUInfo *ObitHostDFTCreate(void);
+int ObitHostDFTInit(GPUInfo *g, const ObitHostUVDesc *uv, const ObitHostSkyModel *sm, int nVisPIO);
+int ObitHostDFTSetMod(GPUInfo *g, const ObitHostSkyModel *sm, const float *comps, int lcomp, int ncomp);
+int ObitHostDFTCalc(GPUInfo *g, float *buffer, long long nvis);
+void ObitHostDFTShutdown(GPUInfo *g);
+
+/* ---- gridding (ObitGPUGrid) ---- */
+/* Spheroidal GCF table of ConvFunc(in,5): 7 cells x 100 points, as the
+ * ObitUVGrid member convfn (src/ObitUVGrid.c:1352-1415); out[708]. */
+void ObitHostConvFuncSph(float *convfn);
+CUDAGridInfo *ObitHostGridCreate(int nfacet, int nGPU, const int *GPUs, const ObitHostUVDesc *uv,
+                                 int nplane, int doBuff, int nVisPIO);
+/* Common set-up (first call, facet 0) and per-facet set-up.  planeOfChan
+ * (nchan*nif entries, may be NULL) gives each channel's plane; sigma1..3
+ * may be NULL. */
+int ObitHostGridSetFacet(CUDAGridInfo *c, int ifacet, const ObitHostFacet *f,
+                         const ObitHostUVDesc *uv, const int *planeOfChan,
+                         const float *sigma1, const float *sigma2, const float *sigma3);
+int ObitHostGridBuffer(CUDAGridInfo *c, const float *buffer, long long nvis);
+int ObitHostGridFlip(CUDAGridInfo *c);
+/* ObitGPUGrid2Host: copy back, trim and centre; out[nplane][ny][nx/2+1][2]. */
+int ObitHostGrid2Host(CUDAGridInfo *c, int ifacet, float *out);
+void ObitHostGridShutdown(CUDAGridInfo *c);
+
+/* Coarse-bin plane of every channel as ObitThreadGridSetupMF assigns it
+ * (channels [bCh, eCh) of each bin, the last channel of each bin left out:
+ * src/ObitThreadGrid.c:553-556) when cpuParity is 1, or every channel in
+ * its bin (ObitGPUGrid.c:471-478) when 0.  bChan/eChan are 0-relative and
+ * inclusive per bin; unassigned channels get nplane+100. */
+void ObitHostPlanesFromBins(int nchanIF, int nbin, const int *bChan, const int *eChan,
+                            int cpuParity, int *planeOfChan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,285 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
+as the checker and by bench.py for the CPU-baseline timing.  The product
+package (obit_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class DftDesc(C.Structure):
+    _fields_ = [
+        ("nrow", C.c_int), ("lrec", C.c_int), ("nrparm", C.c_int),
+        ("ilocu", C.c_int), ("ilocv", C.c_int), ("ilocw", C.c_int),
+        ("nchan", C.c_int), ("nif", C.c_int), ("nstok", C.c_int),
+        ("incs", C.c_int), ("incf", C.c_int), ("incif", C.c_int),
+        ("kincf", C.c_int), ("kincif", C.c_int),
+        ("fscale", C.POINTER(C.c_float)), ("freqArr", C.POINTER(C.c_double)),
+        ("startChan", C.c_int), ("numChan", C.c_int), ("startIF", C.c_int), ("numIF", C.c_int),
+        ("startPoln", C.c_int), ("numPoln", C.c_int),
+        ("modType", C.c_int), ("nSpecTerm", C.c_int),
+        ("lcomp", C.c_int), ("ncomp", C.c_int), ("comps", C.POINTER(C.c_float)),
+        ("smRefFreq", C.c_double),
+        ("isMF", C.c_int), ("specIndex", C.POINTER(C.c_int)), ("specFreq", C.POINTER(C.c_double)),
+        ("doDivide", C.c_int), ("doReplace", C.c_int), ("doFlip", C.c_int),
+        ("stokFactor", C.c_float),
+    ]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("nrow", C.c_int), ("lrec", C.c_int), ("nrparm", C.c_int), ("nchanIF", C.c_int),
+        ("freqArr", C.POINTER(C.c_float)),
+        ("convWidth", C.c_int), ("convNperCell", C.c_int),
+        ("convgfn", C.POINTER(C.c_float)),
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("uscale", C.c_float), ("vscale", C.c_float),
+        ("shift", C.c_float * 3), ("rot", C.c_float * 9),
+        ("guardu", C.c_float), ("guardv", C.c_float),
+        ("maxBL", C.c_float), ("minBL", C.c_float), ("bmTaper", C.c_float),
+        ("isBeam", C.c_int),
+        ("bChan", C.c_int), ("eChan", C.c_int),
+        ("sigma1", C.POINTER(C.c_float)), ("sigma2", C.POINTER(C.c_float)),
+        ("sigma3", C.POINTER(C.c_float)),
+    ]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        P = C.c_void_p
+        lib.orc_sincos_cephes.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        lib.orc_exp_cephes.argtypes = [C.c_float]
+        lib.orc_exp_cephes.restype = C.c_float
+        lib.orc_sincos_vec.argtypes = [C.c_int, P, P, P]
+        lib.orc_exp_vec.argtypes = [C.c_int, P, P]
+        lib.orc_dft_obit.argtypes = [C.POINTER(DftDesc), P, C.c_int, C.c_int]
+        lib.orc_dft_obit_threaded.argtypes = [C.POINTER(DftDesc), P, C.c_int]
+        lib.orc_dft_model_fp64.argtypes = [C.POINTER(DftDesc), P, P, P, C.c_int, C.c_int, C.c_int]
+        lib.orc_convfunc_sph.argtypes = [P, P]
+        lib.orc_sphfn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float]
+        lib.orc_sphfn.restype = C.c_float
+        lib.orc_grid_prep.argtypes = [C.POINTER(GridDesc), P, C.c_int, C.c_int, P, P, P, P, P]
+        lib.orc_grid_obit.argtypes = [C.POINTER(GridDesc), P, C.c_int, C.c_int, P]
+        lib.orc_grid_fp64.argtypes = [C.POINTER(GridDesc), P, C.c_int, C.c_int, P]
+        lib.orc_grid_flip.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        lib.orc_grid_flip_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        lib.orc_grid_merge.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
+        lib.orc_grid_obit_threaded.argtypes = [C.POINTER(GridDesc), P, C.c_int, C.c_int, P]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------ math
+
+def sincos(x: float):
+    s, c = C.c_float(), C.c_float()
+    load().orc_sincos_cephes(C.c_float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def sincos_vec(x: np.ndarray):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    load().orc_sincos_vec(len(x), _p(x), _p(s), _p(c))
+    return s, c
+
+
+def exp_vec(x: np.ndarray):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    load().orc_exp_vec(len(x), _p(x), _p(y))
+    return y
+
+
+# ------------------------------------------------------------------ DFT
+
+class _Keep(list):
+    pass
+
+
+def dft_desc(uv, sky, op="sub", stokFactor=1.0, doFlip=False, startChan=0, numChan=None,
+             startIF=0, numIF=None, startPoln=0, numPoln=None, nrow=None):
+    keep = _Keep()
+    d = DftDesc()
+    d.nrow = uv.nrow if nrow is None else nrow
+    d.lrec, d.nrparm = uv.lrec, uv.nrparm
+    d.ilocu, d.ilocv, d.ilocw = uv.ilocu, uv.ilocv, uv.ilocw
+    d.nchan, d.nif, d.nstok = uv.nchan, uv.nif, uv.nstok
+    d.incs, d.incf, d.incif = uv.incs, uv.incf, uv.incif
+    d.kincf, d.kincif = 1, uv.nchan
+    fs = np.ascontiguousarray(uv.fscale, np.float32)
+    fa = np.ascontiguousarray(uv.freqArr, np.float64)
+    keep += [fs, fa]
+    d.fscale = fs.ctypes.data_as(C.POINTER(C.c_float))
+    d.freqArr = fa.ctypes.data_as(C.POINTER(C.c_double))
+    d.startChan = startChan
+    d.numChan = uv.nchan - startChan if numChan is None else numChan
+    d.startIF = startIF
+    d.numIF = uv.nif - startIF if numIF is None else numIF
+    d.startPoln = startPoln
+    d.numPoln = min(2, uv.nstok) if numPoln is None else numPoln
+    d.modType = sky.modType
+    d.nSpecTerm = sky.nSpecTerm
+    comps = np.ascontiguousarray(sky.comps, np.float32)
+    keep.append(comps)
+    d.lcomp, d.ncomp = comps.shape[1], comps.shape[0]
+    d.comps = comps.ctypes.data_as(C.POINTER(C.c_float))
+    d.smRefFreq = sky.imgRefFreq if sky.imgRefFreq > 0 else uv.freq
+    if sky.specFreq is not None:
+        sf = np.ascontiguousarray(sky.specFreq, np.float64)
+        si = np.ascontiguousarray(sky.specIndex, np.int32)
+        keep += [sf, si]
+        d.isMF = 1
+        d.specFreq = sf.ctypes.data_as(C.POINTER(C.c_double))
+        d.specIndex = si.ctypes.data_as(C.POINTER(C.c_int))
+    d.doDivide = int(op == "div")
+    d.doReplace = int(op == "repl")
+    d.doFlip = int(doFlip)
+    d.stokFactor = stokFactor
+    return d, keep
+
+
+def dft_obit(uv, sky, buffer=None, nthreads=1, **kw):
+    """Obit's CPU DFT (restated) applied in place to a copy of uv.buffer."""
+    buf = np.array(uv.buffer if buffer is None else buffer, dtype=np.float32, copy=True, order="C")
+    d, keep = dft_desc(uv, sky, nrow=buf.shape[0], **kw)
+    load().orc_dft_obit_threaded(C.byref(d), _p(buf), nthreads)
+    return buf
+
+
+def dft_model_fp64(uv, sky, rows=None, nthreads=8, **kw):
+    """FP64 model visibility [nrow, nif, nchan] (complex128)."""
+    buf = uv.buffer if rows is None else np.ascontiguousarray(uv.buffer[rows])
+    buf = np.ascontiguousarray(buf, dtype=np.float32)
+    d, keep = dft_desc(uv, sky, nrow=buf.shape[0], **kw)
+    fr = np.ascontiguousarray(uv.freqArr / uv.freq, np.float64)
+    out = np.zeros((buf.shape[0], uv.nif, uv.nchan, 2), np.float64)
+    load().orc_dft_model_fp64(C.byref(d), _p(buf), _p(fr), _p(out), 0, buf.shape[0], nthreads)
+    return out[..., 0] + 1j * out[..., 1]
+
+
+# ------------------------------------------------------------------ gridding
+
+def convfunc():
+    cf = np.zeros(708, np.float32)
+    cg = np.zeros(708, np.float32)
+    load().orc_convfunc_sph(_p(cf), _p(cg))
+    return cf, cg
+
+
+def grid_desc(uv, facet, bChan=0, eChan=None, guardband=0.4, blmax=0.0, blmin=0.0,
+              bmTaper=0.0, isBeam=False, sigma=None):
+    keep = _Keep()
+    g = GridDesc()
+    g.nrow, g.lrec, g.nrparm = uv.nrow, uv.lrec, uv.nrparm
+    g.nchanIF = uv.nchan * uv.nif
+    fs = np.ascontiguousarray(uv.fscale, np.float32)
+    _, cg = convfunc()
+    keep += [fs, cg]
+    g.freqArr = fs.ctypes.data_as(C.POINTER(C.c_float))
+    g.convWidth, g.convNperCell = 7, 100
+    g.convgfn = cg.ctypes.data_as(C.POINTER(C.c_float))
+    g.nx, g.ny = facet.nx, facet.ny
+    # ObitUVGridSetup (src/ObitUVGrid.c:275-280): cdelt is FP32, the cell in
+    # radians is rounded to FP32, the scales are formed in double then FP32
+    dg = np.pi / 180.0
+    cell = np.float32(dg * float(np.float32(facet.cell_deg)))
+    g.uscale = np.float32(facet.nx * abs(float(cell)))
+    g.vscale = np.float32(-facet.ny * abs(float(cell)))
+    for k in range(3):
+        g.shift[k] = 0.0 if isBeam else float(facet.shift[k])
+    for k in range(9):
+        g.rot[k] = float(facet.rot[k])
+    # guardband*nx in float (ObitThreadGrid.c:328-329)
+    g.guardu = np.float32(np.float32(guardband) * np.float32(facet.nx))
+    g.guardv = np.float32(np.float32(guardband) * np.float32(facet.ny))
+    g.maxBL = blmax if blmax > 0 else 1.0e12
+    g.minBL = blmin
+    g.bmTaper = bmTaper
+    g.isBeam = int(isBeam)
+    g.bChan = bChan
+    g.eChan = g.nchanIF if eChan is None else eChan
+    if sigma is not None:
+        s = [np.ascontiguousarray(x, np.float32) for x in sigma]
+        keep += s
+        g.sigma1, g.sigma2, g.sigma3 = (x.ctypes.data_as(C.POINTER(C.c_float)) for x in s)
+    return g, keep
+
+
+def grid_prep(uv, facet, rows=None, **kw):
+    buf = uv.buffer if rows is None else np.ascontiguousarray(uv.buffer[rows])
+    g, keep = grid_desc(uv, facet, **kw)
+    g.nrow = buf.shape[0]
+    n = buf.shape[0] * g.nchanIF
+    iu, iv, itu, itv = (np.zeros(n, np.int32) for _ in range(4))
+    vis = np.zeros(2 * n, np.float32)
+    load().orc_grid_prep(C.byref(g), _p(np.ascontiguousarray(buf, np.float32)), 0, buf.shape[0],
+                         _p(iu), _p(iv), _p(itu), _p(itv), _p(vis))
+    return iu, iv, itu, itv, vis
+
+
+def grid_size(facet):
+    return 2 * (1 + 3 + facet.nx // 2) * facet.ny
+
+
+def grid_fp64(uv, facet, flip=True, **kw):
+    g, keep = grid_desc(uv, facet, **kw)
+    buf = np.ascontiguousarray(uv.buffer, np.float32)
+    grid = np.zeros(grid_size(facet), np.float64)
+    load().orc_grid_fp64(C.byref(g), _p(buf), 0, buf.shape[0], _p(grid))
+    if flip:
+        load().orc_grid_flip_fp64(_p(grid), facet.nx, facet.ny, 7)
+    return grid
+
+
+def grid_obit(uv, facet, flip=True, **kw):
+    g, keep = grid_desc(uv, facet, **kw)
+    buf = np.ascontiguousarray(uv.buffer, np.float32)
+    grid = np.zeros(grid_size(facet), np.float32)
+    load().orc_grid_obit(C.byref(g), _p(buf), 0, buf.shape[0], _p(grid))
+    if flip:
+        load().orc_grid_flip(_p(grid), facet.nx, facet.ny, 7)
+    return grid
+
+
+def grid_obit_threaded(uv, facet, nGpI=4, nthreads=4, **kw):
+    """ObitThreadGrid decomposition; returns complex64 [ny, nx/2+1] merged."""
+    g, keep = grid_desc(uv, facet, **kw)
+    buf = np.ascontiguousarray(uv.buffer, np.float32)
+    out = np.zeros((facet.ny, facet.nx // 2 + 1, 2), np.float32)
+    load().orc_grid_obit_threaded(C.byref(g), _p(buf), nGpI, nthreads, _p(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def trim_centre(grid, facet, dtype=np.complex128):
+    """Padded grid -> [ny, nx/2+1] with v halves swapped (ThreadMerge)."""
+    ny, nx = facet.ny, facet.nx
+    lrow = 2 * (1 + nx // 2 + 3)
+    g = grid[:lrow * ny].reshape(ny, lrow // 2, 2)
+    g = g[:, 3:3 + nx // 2 + 1, 0] + 1j * g[:, 3:3 + nx // 2 + 1, 1]
+    return np.roll(g, ny // 2, axis=0).astype(dtype)

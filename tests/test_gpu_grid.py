@@ -1,0 +1,139 @@
+"""GPU gridding parity: device sample arithmetic vs the CPU restatement
+(bit-exact indices, taps and weighted values), gridded planes vs the FP64
+oracle on the same samples, flip and trim/centre vs Obit's CPU routines."""
+import numpy as np
+import pytest
+
+from obit_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _facets_cfg4(n=3, nx=1024):
+    return synth.make_facets(n, nx, 0.4 / 3600.0, 0.3 * np.pi / 180.0)
+
+
+@pytest.fixture(scope="module")
+def uv4():
+    uv, _, _ = synth.workload(4, rows=384)
+    return uv
+
+
+def _gpu_grid(uv, facets, **kw):
+    from obit_amd.gridder import GPUGrid
+    return GPUGrid(uv, facets, **kw)
+
+
+@pytest.mark.parametrize("ifacet", [0, 1, 2])
+def test_prep_bit_exact(oracle, uv4, ifacet):
+    facets = _facets_cfg4()
+    with _gpu_grid(uv4, facets) as gg:
+        iu, iv, itu, itv, vis = gg.prep_debug(ifacet, uv4.buffer)
+    oiu, oiv, oitu, oitv, ovis = oracle.grid_prep(uv4, facets[ifacet])
+    ok = ovis.reshape(-1, 2).any(axis=1)
+    print(f"facet {ifacet}: {ok.sum()} valid samples of {ok.size}", flush=True)
+    assert ok.sum() > 0.5 * ok.size
+    for name, a, b in (("iu", iu, oiu), ("iv", iv, oiv), ("itu", itu, oitu), ("itv", itv, oitv)):
+        bad = np.flatnonzero(a[ok] != b[ok])
+        assert bad.size == 0, f"{name}: {bad.size} mismatches, first {a[ok][bad[:5]]} vs {b[ok][bad[:5]]}"
+    assert np.array_equal(vis.view(np.uint32), ovis.view(np.uint32)), "weighted visibilities differ"
+
+
+def _planes_vs_oracle(oracle, uv, facets, gg, plane_bins=None, **okw):
+    worst = 0.0
+    for i, f in enumerate(facets):
+        g = gg.to_host(i)
+        nplane = g.shape[0]
+        for p in range(nplane):
+            kw = dict(okw)
+            if plane_bins is not None:
+                kw["bChan"], kw["eChan"] = plane_bins[p]
+            ref = oracle.trim_centre(oracle.grid_fp64(uv, f, **kw), f)
+            err = np.abs(g[p] - ref).max() / np.abs(ref).max()
+            worst = max(worst, err)
+    return worst
+
+
+def test_grid_single_plane(oracle, uv4):
+    facets = _facets_cfg4()
+    with _gpu_grid(uv4, facets) as gg:
+        gg.grid(uv4.buffer)
+        gg.flip()
+        err = _planes_vs_oracle(oracle, uv4, facets, gg)
+    print(f"max |dG|/peak {err:.3e}", flush=True)
+    assert err < 2e-6
+
+
+def test_grid_unflipped_and_flip(oracle, uv4):
+    """Planes before and after the device flip against the oracle grid
+    before and after ThreadFlip."""
+    f = _facets_cfg4(1)
+    with _gpu_grid(uv4, f) as gg:
+        gg.grid(uv4.buffer)
+        raw = gg.to_host(0)[0]
+        gg.flip()
+        flipped = gg.to_host(0)[0]
+    ref_raw = oracle.trim_centre(oracle.grid_fp64(uv4, f[0], flip=False), f[0])
+    ref_flip = oracle.trim_centre(oracle.grid_fp64(uv4, f[0], flip=True), f[0])
+    peak = np.abs(ref_flip).max()
+    assert np.abs(raw - ref_raw).max() < 2e-6 * peak
+    assert np.abs(flipped - ref_flip).max() < 2e-6 * peak
+    assert np.abs(ref_flip - ref_raw).max() > 1e-3 * peak  # the flip did something
+
+
+def test_grid_mf_bins_cpu_parity(oracle):
+    """8 coarse bins of 32 channels, the CPU's exclusive upper bound (last
+    channel of each bin left out), 2 facets."""
+    from obit_amd.gridder import plane_of_channel
+    uv, _, _ = synth.workload(4, rows=192)
+    facets = _facets_cfg4(2)
+    b, e = synth.coarse_bins(256, 8)
+    poc = plane_of_channel(256, b, e, cpu_parity=True)
+    assert (poc[e] == 8 + 100).all() and (poc[b] == np.arange(8)).all()
+    with _gpu_grid(uv, facets, nplane=8, plane_of_chan=poc) as gg:
+        gg.grid(uv.buffer)
+        gg.flip()
+        err = _planes_vs_oracle(oracle, uv, facets, gg, plane_bins=[(int(b[p]), int(e[p])) for p in range(8)])
+    print(f"mf max |dG|/peak {err:.3e}", flush=True)
+    assert err < 2e-6
+
+
+def test_grid_beam_and_taper(oracle, uv4):
+    f = _facets_cfg4(1)
+    taper = -1.0e-10
+    with _gpu_grid(uv4, f, beam=True, bmTaper=taper) as gg:
+        gg.grid(uv4.buffer)
+        gg.flip()
+        g = gg.to_host(0)[0]
+    ref = oracle.trim_centre(oracle.grid_fp64(uv4, f[0], isBeam=True, bmTaper=taper), f[0])
+    err = np.abs(g - ref).max() / np.abs(ref).max()
+    print(f"beam+taper max |dG|/peak {err:.3e}", flush=True)
+    # expf on the device and in libm may differ by an ulp
+    assert err < 5e-6
+
+
+def test_grid_device_path_matches_buffers(uv4):
+    """One resident-device pass == the per-buffer (nVisPIO) path."""
+    from obit_amd import _lib
+    lib = _lib.load()
+    f = _facets_cfg4(2)
+    with _gpu_grid(uv4, f, nVisPIO=100) as gg:
+        gg.grid(uv4.buffer)
+        gg.flip()
+        a = [gg.to_host(i) for i in range(2)]
+    import ctypes as C
+    dptr = C.c_void_p()
+    nbytes = uv4.buffer.nbytes
+    _lib.check(lib.ObitHIPMalloc(C.byref(dptr), nbytes))
+    try:
+        _lib.check(lib.ObitHIPMemcpyH2D(dptr, _lib.ptr(uv4.buffer), nbytes))
+        with _gpu_grid(uv4, f) as gg:
+            gg.grid_device(dptr.value, uv4.nrow)
+            gg.flip()
+            b = [gg.to_host(i) for i in range(2)]
+            c = [gg.to_host_device_path(i) for i in range(2)]
+    finally:
+        lib.ObitHIPFree(dptr)
+    for x, y, z in zip(a, b, c):
+        assert np.abs(x - y).max() <= 1e-6 * np.abs(x).max()
+        assert np.array_equal(y, z)
