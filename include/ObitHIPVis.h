@@ -37,6 +37,10 @@ int ObitHIPDeviceSynchronize(void);
 float ObitHIPEventElapsedMs(int *start, int *stop);
 int ObitHIPDeviceInfo(int device, char *name, int nameLen, int *nCU, int *clockKHz,
                       long long *memBytes);
+/* Plain device-to-device copy kernel on stream (bytes a multiple of 16):
+ * the known-size reference against which HBM traffic counters are
+ * calibrated. */
+int ObitHIPCopyKernel(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ---- DFT sky model ---- */
 

@@ -118,6 +118,7 @@ _PROTOS = {
     "ObitHIPDeviceSynchronize": (_I, []),
     "ObitHIPEventElapsedMs": (_F, [_P, _P]),
     "ObitHIPDeviceInfo": (_I, [_I, C.c_char_p, _I, _IP, _IP, C.POINTER(_LL)]),
+    "ObitHIPCopyKernel": (_I, [_P, _P, C.c_size_t, _P]),
     "ObitHIPSkyModelDFTSetFreq": (_I, [_P, _DP, _I, C.c_double]),
     "ObitHIPSkyModelDFTSetSpecRef": (_I, [_P, _DP, _I]),
     "ObitHIPSkyModelDFTSetVis": (_I, [_P, _P, _LL]),

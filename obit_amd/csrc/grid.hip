@@ -733,12 +733,8 @@ void ObitCUDAGridInit(CUDAGridInfo *cudaInfo, long iGPU) {
     FacetInfo *fi = cudaInfo->h_facetInfo[i];
     const size_t bytes = (size_t)fi->sizeGrid * nplane * sizeof(float);
     if (!fi->d_grid) OBIT_HIP_CHECK(hipMalloc(&fi->d_grid, bytes));
-    if (!fi->h_grid) {
-      void *p = nullptr;
-      OBIT_HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
-      fi->h_grid = (float *)p;
-      std::memset(fi->h_grid, 0, bytes);
-    }
+    // the pinned host copy (h_grid) is allocated by ObitCUDAGrid2CPU on
+    // first use, so grids that stay on the device cost no host memory
     OBIT_HIP_CHECK(hipMemsetAsync(fi->d_grid, 0, bytes, gs->stream));
   }
   OBIT_HIP_CHECK(hipStreamSynchronize(gs->stream));
@@ -788,6 +784,12 @@ void ObitCUDAGrid2CPU(CUDAGridInfo *cudaInfo, long ifacet) {
   GpuState *gs = obit::grid::get_gpu(cudaInfo, iGPU, false);
   if (gs) OBIT_HIP_CHECK(hipSetDevice(gs->device));
   const size_t bytes = (size_t)fi->sizeGrid * cudaInfo->h_gridInfo->nplane * sizeof(float);
+  if (!fi->h_grid) {
+    void *p = nullptr;
+    OBIT_HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    fi->h_grid = (float *)p;
+  }
+  if (gs) OBIT_HIP_CHECK(hipStreamSynchronize(gs->stream));
   OBIT_HIP_CHECK(hipMemcpy(fi->h_grid, fi->d_grid, bytes, hipMemcpyDeviceToHost));
 }
 

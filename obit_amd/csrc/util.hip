@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 
 namespace obit {
@@ -213,6 +214,26 @@ int ObitHIPDeviceInfo(int device, char *name, int nameLen, int *nCU, int *clockK
   if (nCU) *nCU = prop.multiProcessorCount;
   if (clockKHz) *clockKHz = prop.clockRate;
   if (memBytes) *memBytes = (long long)prop.totalGlobalMem;
+  return obit::kOK;
+}
+
+// Plain device copy kernel (16-byte vectors): the known-size reference the
+// HBM-traffic counters are calibrated against.
+__global__ void k_copy16(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int ObitHIPCopyKernel(void *dst, const void *src, size_t bytes, void *stream) {
+  if ((bytes & 15) || ((size_t)dst & 15) || ((size_t)src & 15))
+    return obit::fail(obit::kInvalid, "ObitHIPCopyKernel: 16-byte aligned sizes only");
+  const size_t n = bytes / 16;
+  if (n == 0) return obit::kOK;
+  const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                     (const float4 *)src, (float4 *)dst, n);
+  OBIT_HIP_TRY(hipGetLastError());
   return obit::kOK;
 }
 
