@@ -206,6 +206,7 @@ def main():
     ap.add_argument("--grid-steps", type=int, default=3)
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid-only", action="store_true", help="time the gridding workload alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -218,6 +219,9 @@ def main():
     _lib.check(lib.ObitHIPDeviceInfo(0, name, 64, C.byref(ncu), C.byref(clk), C.byref(mem)), "device")
     log(f"device {name.value.decode()} {ncu.value} CUs {clk.value / 1e3:.0f} MHz {mem.value / 2**30:.0f} GiB")
 
+    if args.grid_only:
+        print(json.dumps({"gridding": bench_grid(lib, args.grid_steps)}), flush=True)
+        return
     dft, uv, sky = bench_dft(lib, args.steps, args.warmup, args.sample_rows)
     grid = None if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)

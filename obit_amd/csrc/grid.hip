@@ -22,8 +22,6 @@
 #include "ObitCUDAUtil.h"
 #include "ObitHIPVis.h"
 
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -35,7 +33,7 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 32;        // tile height in grid rows
+constexpr int TV = 64;        // tile height in grid rows
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kThreads = 256;
 
@@ -267,19 +265,57 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
 
 // ---------------- tile gridding ----------------
 
-// One workgroup per (facet, plane, tile) list.  LDS holds the tile as
-// separate real and imaginary planes (TV x TU floats each) and the GCF.
-__global__ __launch_bounds__(kThreads) void k_grid_tiles(const float *vis, const FacetDev *facets,
-                                                         const int *listFacet, long long nlist,
-                                                         Common cm, const int *counts,
-                                                         const int *offsets, const int *lists) {
+constexpr int kW = 7;          // GCF support in cells (checked on the host)
+constexpr int kChunk = 512;    // tracks per work item
+
+// Add a lane's 7x7 register window (base cell bu, bv) into the LDS tile,
+// dropping cells outside the tile, and clear the window.
+__device__ __forceinline__ void flush_window(float (&ar)[kW * kW], float (&ai)[kW * kW], int bu,
+                                             int bv, int col0, int row0, float *s_re, float *s_im) {
+  const int c0 = bu - col0, r0 = bv - row0;
+#pragma unroll
+  for (int jv = 0; jv < kW; ++jv) {
+    const int rr = r0 + jv;
+    const bool rok = rr >= 0 && rr < TV;
+#pragma unroll
+    for (int ju = 0; ju < kW; ++ju) {
+      const int cc = c0 + ju;
+      if (rok && cc >= 0 && cc < TU) {
+        atomicAdd(&s_re[rr * TU + cc], ar[jv * kW + ju]);
+        atomicAdd(&s_im[rr * TU + cc], ai[jv * kW + ju]);
+      }
+      ar[jv * kW + ju] = 0.0f;
+      ai[jv * kW + ju] = 0.0f;
+    }
+  }
+}
+
+// One workgroup per work item = (tile list, chunk of <= kChunk tracks).
+// Each lane walks one track (row) through the plane's channels in
+// frequency order, keeping the 7x7 footprint of the current base cell in
+// registers; consecutive channels of a track usually share a base cell,
+// so the window is flushed into the LDS tile only when the base cell
+// moves.  A tile whose list fits one chunk is added into the grid with
+// plain loads/stores; chunks of a longer list add with device atomics.
+__global__ __launch_bounds__(kThreads) void k_grid_chunks(const float *vis, const FacetDev *facets,
+                                                          const int *listFacet, const int2 *items,
+                                                          long long nitems, Common cm,
+                                                          const int *counts, const int *offsets,
+                                                          const int *lists) {
   __shared__ float s_re[TV * TU];
   __shared__ float s_im[TV * TU];
-  __shared__ float s_gcf[kGcf];
-  for (int i = threadIdx.x; i < kGcf; i += kThreads) s_gcf[i] = cm.gcf[i];
-  for (long long L = blockIdx.x; L < nlist; L += gridDim.x) {
+  __shared__ float4 s_gcf[101 * 2];  // row it: taps 0..6 (+ one zero pad)
+  for (int i = threadIdx.x; i < 101 * 8; i += kThreads) {
+    const int it = i >> 3, c = i & 7;
+    reinterpret_cast<float *>(s_gcf)[i] = (c < kW) ? cm.gcf[kW * it + c] : 0.0f;
+  }
+  for (long long w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int2 item = items[w];
+    const long long L = item.x;
     const int n = counts[L];
-    if (n == 0) continue;
+    const int e0 = item.y * kChunk;
+    const int e1 = min(n, e0 + kChunk);
+    const bool multi = n > kChunk;
     const int fi = listFacet[L];
     const FacetDev &f = facets[fi];
     const long long rel = L - f.listBase;
@@ -294,36 +330,42 @@ __global__ __launch_bounds__(kThreads) void k_grid_tiles(const float *vis, const
     const int *lst = lists + offsets[L];
     const int npc = cm.pcount[p];
     const int *pch = cm.pchan + (size_t)p * cm.maxpc;
-    const int W = cm.convWidth;
-    for (int e = threadIdx.x; e < n; e += kThreads) {
+    for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
       const long long r = lst[e];
       const float *row = vis + r * (long long)cm.lenvis;
       float uu, vv, ww;
       rotate_uvw(f, row, uu, vv, ww);
+      float ar[kW * kW], ai[kW * kW];
+#pragma unroll
+      for (int j = 0; j < kW * kW; ++j) { ar[j] = 0.0f; ai[j] = 0.0f; }
+      int bu = 0, bv = 0;
+      bool have = false;
       for (int k = 0; k < npc; ++k) {
         const Sample s = prep_sample(f, cm, row, uu, vv, ww, pch[k]);
         if (s.vr == 0.0f && s.vi == 0.0f) continue;
-        const int c0 = s.iu - col0, r0 = s.iv - row0;
-        if (c0 + W <= 0 || c0 >= TU || r0 + W <= 0 || r0 >= TV) continue;
-        const float *cu = s_gcf + W * s.itu;
-        const float *cv = s_gcf + W * s.itv;
-        for (int jv = 0; jv < W; ++jv) {
-          const int rr = r0 + jv;
-          if (rr < 0 || rr >= TV) continue;
-          const float cvv = cv[jv];
-          if (cvv == 0.0f) continue;
-          for (int ju = 0; ju < W; ++ju) {
-            const int cc = c0 + ju;
-            if (cc < 0 || cc >= TU) continue;
-            const float cfn = cu[ju] * cvv;
-            atomicAdd(&s_re[rr * TU + cc], s.vr * cfn);
-            atomicAdd(&s_im[rr * TU + cc], s.vi * cfn);
+        if (!have || s.iu != bu || s.iv != bv) {
+          if (have) flush_window(ar, ai, bu, bv, col0, row0, s_re, s_im);
+          bu = s.iu;
+          bv = s.iv;
+          have = true;
+        }
+        const float4 u0 = s_gcf[2 * s.itu], u1 = s_gcf[2 * s.itu + 1];
+        const float4 v0 = s_gcf[2 * s.itv], v1 = s_gcf[2 * s.itv + 1];
+        const float cu[kW] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z};
+        const float cv[kW] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z};
+#pragma unroll
+        for (int jv = 0; jv < kW; ++jv) {
+          const float wr = s.vr * cv[jv], wi = s.vi * cv[jv];
+#pragma unroll
+          for (int ju = 0; ju < kW; ++ju) {
+            ar[jv * kW + ju] = fmaf(wr, cu[ju], ar[jv * kW + ju]);
+            ai[jv * kW + ju] = fmaf(wi, cu[ju], ai[jv * kW + ju]);
           }
         }
       }
+      if (have) flush_window(ar, ai, bu, bv, col0, row0, s_re, s_im);
     }
     __syncthreads();
-    // add the tile into the grid (each cell belongs to this workgroup only)
     float *g = f.grid + (size_t)p * f.sizeGrid;
     const int ncol = f.lrow / 2;
     for (int i = threadIdx.x; i < TV * TU; i += kThreads) {
@@ -333,8 +375,13 @@ __global__ __launch_bounds__(kThreads) void k_grid_tiles(const float *vis, const
       const float re = s_re[i], im = s_im[i];
       if (re == 0.0f && im == 0.0f) continue;
       float *q = g + (size_t)gr * f.lrow + 2 * gc;
-      q[0] += re;
-      q[1] += im;
+      if (multi) {
+        unsafeAtomicAdd(q, re);
+        unsafeAtomicAdd(q + 1, im);
+      } else {
+        q[0] += re;
+        q[1] += im;
+      }
     }
   }
 }
@@ -415,8 +462,10 @@ struct GpuState {
   long long nlist = 0;
   int *d_counts = nullptr, *d_cursor = nullptr, *d_offsets = nullptr, *d_lists = nullptr;
   long long listCap = 0;
-  void *d_scanTmp = nullptr;
-  size_t scanTmpCap = 0;
+  int2 *d_items = nullptr;
+  long long itemCap = 0;
+  std::vector<int> h_counts, h_offsets;
+  std::vector<int2> h_items;
   // tables
   float *d_gcf = nullptr, *d_freq = nullptr, *d_sig = nullptr, *d_pf = nullptr;
   int *d_pchan = nullptr, *d_pcount = nullptr;
@@ -591,14 +640,6 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   OBIT_HIP_TRY(hipMalloc(&gs->d_counts, sizeof(int) * (gs->nlist + 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_cursor, sizeof(int) * (gs->nlist + 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_offsets, sizeof(int) * (gs->nlist + 1)));
-  size_t tmp = 0;
-  OBIT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, gs->d_counts, gs->d_offsets,
-                                                (int)(gs->nlist + 1), gs->stream));
-  if (tmp > gs->scanTmpCap) {
-    (void)hipFree(gs->d_scanTmp);
-    OBIT_HIP_TRY(hipMalloc(&gs->d_scanTmp, tmp));
-    gs->scanTmpCap = tmp;
-  }
   gs->ready = true;
   return kOK;
 }
@@ -620,28 +661,53 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 0);
   OBIT_HIP_TRY(hipGetLastError());
-  size_t tmp = gs->scanTmpCap;
-  OBIT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(gs->d_scanTmp, tmp, gs->d_counts, gs->d_offsets,
-                                                (int)(nl + 1), gs->stream));
-  int total = 0;
-  OBIT_HIP_TRY(hipMemcpyAsync(&total, gs->d_offsets + nl, sizeof(int), hipMemcpyDeviceToHost, gs->stream));
+  // list sizes to the host: offsets (exclusive scan) and the work items
+  // (list, chunk) that split long lists into kChunk-track pieces
+  gs->h_counts.resize((size_t)nl);
+  OBIT_HIP_TRY(hipMemcpyAsync(gs->h_counts.data(), gs->d_counts, sizeof(int) * nl, hipMemcpyDeviceToHost,
+                              gs->stream));
   OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
-  if (total < 0) return fail(kInvalid, "list size overflow");
+  gs->h_offsets.resize((size_t)nl + 1);
+  gs->h_items.clear();
+  long long total = 0;
+  for (long long L = 0; L < nl; ++L) {
+    gs->h_offsets[L] = (int)total;
+    const int n = gs->h_counts[L];
+    for (int c = 0; c * kChunk < n; ++c) gs->h_items.push_back(make_int2((int)L, c));
+    total += n;
+    if (total > 0x7fffffffLL) return fail(kInvalid, "track lists exceed 2^31 entries");
+  }
+  gs->h_offsets[nl] = (int)total;
+  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_offsets, gs->h_offsets.data(), sizeof(int) * (nl + 1),
+                              hipMemcpyHostToDevice, gs->stream));
   if ((long long)total > gs->listCap) {
     (void)hipFree(gs->d_lists);
     gs->d_lists = nullptr;
-    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int) * (size_t)std::max(total, 1));
-    if (e != hipSuccess) return fail(kNoMemory, "track lists (%d): %s", total, hipGetErrorString(e));
+    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int) * (size_t)std::max<long long>(total, 1));
+    if (e != hipSuccess) return fail(kNoMemory, "track lists (%lld): %s", total, hipGetErrorString(e));
     gs->listCap = total;
   }
+  const long long nitems = (long long)gs->h_items.size();
+  if (nitems > gs->itemCap) {
+    (void)hipFree(gs->d_items);
+    OBIT_HIP_TRY(hipMalloc(&gs->d_items, sizeof(int2) * (size_t)std::max<long long>(nitems, 1)));
+    gs->itemCap = nitems;
+  }
+  if (nitems)
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->d_items, gs->h_items.data(), sizeof(int2) * nitems,
+                                hipMemcpyHostToDevice, gs->stream));
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
   OBIT_HIP_TRY(hipGetLastError());
-  const long long nbG = std::min<long long>(nl, (long long)gs->nCU * 8);
-  hipLaunchKernelGGL(k_grid_tiles, dim3((unsigned)nbG), dim3(kThreads), 0, gs->stream, vis,
-                     gs->d_facets, gs->d_listFacet, nl, gs->cm, gs->d_counts, gs->d_offsets,
-                     gs->d_lists);
-  OBIT_HIP_TRY(hipGetLastError());
+  if (nitems) {
+    const long long nbG = std::min<long long>(nitems, (long long)gs->nCU * 8);
+    hipLaunchKernelGGL(k_grid_chunks, dim3((unsigned)nbG), dim3(kThreads), 0, gs->stream, vis,
+                       gs->d_facets, gs->d_listFacet, (const int2 *)gs->d_items, nitems, gs->cm,
+                       gs->d_counts, gs->d_offsets, gs->d_lists);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  // the host vectors are read by the async copies above
+  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
   return kOK;
 }
 
@@ -809,7 +875,7 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
     (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
-    (void)hipFree(gs->d_scanTmp); (void)hipFree(gs->d_vis);
+    (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis);
     if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
     (void)hipStreamDestroy(gs->stream);
     obit::grid::GridCtx *c = obit::grid::get_ctx(cudaInfo, false);
