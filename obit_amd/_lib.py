@@ -136,6 +136,7 @@ _PROTOS = {
     "ObitHIPGrid2Device": (_I, [_P, _L, _P]),
     "ObitHIPGrid2HostPlanes": (_I, [_P, _L, _P]),
     "ObitHIPGridPrepDebug": (_I, [_P, _L, _P, _LL, _P, _P, _P, _P, _P]),
+    "ObitHIPGridListStats": (_I, [_P, _L, _P]),
     "ObitHIPGridStream": (_P, [_P, _L]),
     # host classes (csrc/host.h)
     "ObitHostDFTCreate": (_P, []),

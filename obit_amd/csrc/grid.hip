@@ -33,9 +33,10 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 64;        // tile height in grid rows
+constexpr int TV = 32;        // tile height in grid rows
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
-constexpr int kThreads = 256;
+constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
+
 
 struct FacetDev {
   float *grid;
@@ -60,7 +61,7 @@ struct Common {
   const int *pcount;    // [nplane]
   const float *pfmin, *pfmax;  // plane frequency extremes
   int maxpc;
-  const float *gcf;     // [708]
+  const float *gcf;     // [101][8]: row it = taps 0..6 of fractional offset it, then 0
 };
 
 // ---------------- device: CPU-exact sample arithmetic ----------------
@@ -123,13 +124,17 @@ struct Sample {
 };
 
 // Rotated uvw of one row for a facet (float, left-to-right, no FMA).
-__device__ __forceinline__ void rotate_uvw(const FacetDev &f, const float *row, float &uu,
-                                           float &vv, float &ww) {
+__device__ __forceinline__ void rotate3(const FacetDev &f, float u, float v, float w, float &uu,
+                                        float &vv, float &ww) {
 #pragma clang fp contract(off)
-  const float u = row[0], v = row[1], w = row[2];
   uu = u * f.rot[0] + v * f.rot[1] + w * f.rot[2];
   vv = u * f.rot[3] + v * f.rot[4] + w * f.rot[5];
   ww = u * f.rot[6] + v * f.rot[7] + w * f.rot[8];
+}
+
+__device__ __forceinline__ void rotate_uvw(const FacetDev &f, const float *row, float &uu,
+                                           float &vv, float &ww) {
+  rotate3(f, row[0], row[1], row[2], uu, vv, ww);
 }
 
 // Cells of a channel at FP32 frequency factor freq (no validity test):
@@ -146,15 +151,15 @@ __device__ __forceinline__ void cells_at(const FacetDev &f, float uu, float vv, 
   ivrow = lround_obit(v) - f.hw + f.ny / 2;
 }
 
-// fast_rot + fast_prep_grid for one (row, channel) of one facet.
-__device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *row, float uu,
-                              float vv, float ww, int ich) {
+// fast_rot + fast_prep_grid for one (row, channel) of one facet, given
+// the channel's (re, im, weight).
+__device__ __forceinline__ Sample prep_vals(const FacetDev &f, const Common &cm, float uu, float vv,
+                                            float ww, int ich, float vre, float vim, float vwt) {
 #pragma clang fp contract(off)
   Sample s;
   const float phaseSign = (uu <= 0.0f) ? -1.0f : 1.0f;
   const float freqFact = phaseSign * cm.freq[ich];
   float u = uu * freqFact, v = vv * freqFact, w = ww * freqFact;
-  const float *vis = row + cm.nrparm + ich * 3;
   // fast_rot: phase shift (double phase, reduced, FP32 Cephes sincos)
   float vr, vi;
   if (f.doBeam) {
@@ -170,13 +175,13 @@ __device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *
     const float ph = (float)(dphase - (double)iphase * twopi);
     float sn, cs;
     cephes_sincos(ph, &sn, &cs);
-    const float vvr = vis[0];
-    const float vvi = vis[1] * phaseSign;
+    const float vvr = vre;
+    const float vvi = vim * phaseSign;
     vr = vvr * cs - vvi * sn;
     vi = vvr * sn + vvi * cs;
   }
   // fast_prep_grid
-  float vw = vis[2];
+  float vw = vwt;
   bool want = (vw > 0.0f) && (u < f.guardu) && (fabsf(v) < f.guardv);
   const float BL2 = u * u + v * v;
   if ((f.maxBL2 > 0.0f) && want) want = f.maxBL2 > BL2;
@@ -212,6 +217,12 @@ __device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *
   s.iu = iu;
   s.iv = iv - f.hw + f.ny / 2;
   return s;
+}
+
+__device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *row, float uu,
+                              float vv, float ww, int ich) {
+  const float *vis = row + cm.nrparm + ich * 3;
+  return prep_vals(f, cm, uu, vv, ww, ich, vis[0], vis[1], vis[2]);
 }
 
 // ---------------- binning ----------------
@@ -267,49 +278,64 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
 
 constexpr int kW = 7;          // GCF support in cells (checked on the host)
 constexpr int kChunk = 512;    // tracks per work item
+constexpr int kWaves = 2;      // waves per workgroup, each with a private tile
+// Tile row stride in floats, 7 mod 64: the 7x7 footprint of any base cell
+// then falls on 49 distinct banks (row tjv, column tju -> bank offset
+// 7*tjv + tju = the tap's lane number).
+constexpr int kS = TU + 7;
+constexpr int kPark = ((TV * kS + 63) / 64) * 64;  // 64 parking slots after the tile rows
+constexpr int kPlane = kPark + 64;                 // floats per re / im plane
+constexpr int kStage = 24;     // floats per staged sample: 8 u-weights, 8 (re, im) v-products
+constexpr int kGroup = 4;      // staged samples read ahead per step
+constexpr int kStageN = 64 + kGroup;  // staged samples per wave, with read-ahead slack
+constexpr int kNoCell = 0x7fffffff;
+static_assert(kS % 64 == 7 && kS >= TU, "tile stride must be 7 mod 64");
+static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 
-// Add a lane's 7x7 register window (base cell bu, bv) into the LDS tile,
-// dropping cells outside the tile, and clear the window.
-__device__ __forceinline__ void flush_window(float (&ar)[kW * kW], float (&ai)[kW * kW], int bu,
-                                             int bv, int col0, int row0, float *s_re, float *s_im) {
-  const int c0 = bu - col0, r0 = bv - row0;
-#pragma unroll
-  for (int jv = 0; jv < kW; ++jv) {
-    const int rr = r0 + jv;
-    const bool rok = rr >= 0 && rr < TV;
-#pragma unroll
-    for (int ju = 0; ju < kW; ++ju) {
-      const int cc = c0 + ju;
-      if (rok && cc >= 0 && cc < TU) {
-        atomicAdd(&s_re[rr * TU + cc], ar[jv * kW + ju]);
-        atomicAdd(&s_im[rr * TU + cc], ai[jv * kW + ju]);
-      }
-      ar[jv * kW + ju] = 0.0f;
-      ai[jv * kW + ju] = 0.0f;
-    }
-  }
-}
-
-// One workgroup per work item = (tile list, chunk of <= kChunk tracks).
-// Each lane walks one track (row) through the plane's channels in
-// frequency order, keeping the 7x7 footprint of the current base cell in
-// registers; consecutive channels of a track usually share a base cell,
-// so the window is flushed into the LDS tile only when the base cell
-// moves.  A tile whose list fits one chunk is added into the grid with
-// plain loads/stores; chunks of a longer list add with device atomics.
-__global__ __launch_bounds__(kThreads) void k_grid_chunks(const float *vis, const FacetDev *facets,
-                                                          const int *listFacet, const int2 *items,
-                                                          long long nitems, Common cm,
-                                                          const int *counts, const int *offsets,
-                                                          const int *lists) {
-  __shared__ float s_re[TV * TU];
-  __shared__ float s_im[TV * TU];
-  __shared__ float4 s_gcf[101 * 2];  // row it: taps 0..6 (+ one zero pad)
-  for (int i = threadIdx.x; i < 101 * 8; i += kThreads) {
-    const int it = i >> 3, c = i & 7;
-    reinterpret_cast<float *>(s_gcf)[i] = (c < kW) ? cm.gcf[kW * it + c] : 0.0f;
-  }
-  for (long long w = blockIdx.x; w < nitems; w += gridDim.x) {
+// Work items are (tile list, chunk of <= kChunk tracks).  Each wave takes
+// its own item and owns a private TV x TU tile (separate re / im planes)
+// in LDS, so no tile update needs an atomic.
+//
+// The item's (track, channel) samples are visited 64 at a time, lanes =
+// samples in track-major, channel-minor order; the rows for the next 64
+// are fetched while the current 64 are gridded.  Each sample goes through
+// the CPU gridder's arithmetic (prep_vals); those whose 7x7 footprint
+// touches the tile are compacted, in order, into a per-wave stage as the
+// seven u-weights and the seven products v-weight x visibility, and a
+// 64-bit mask marks the staged samples whose base cell differs from the
+// previous one.  The staged samples are then walked with lanes = the 49
+// taps of the footprint: each tap lane keeps a running sum while the base
+// cell stays put and hands it to its tile cell when the mask says the
+// cell moves.  A hand-over is a read-modify-write whose write is deferred
+// to the next hand-over, so no step waits on an LDS read; all of one
+// hand-over's writes are issued before any of the next one's reads, which
+// keeps successive overlapping footprints exact.  Lanes without a tile
+// cell (the 15 spare lanes, taps off the tile's edge) park in a slot on
+// the bank their cell would have used.  A tile whose list fits one item
+// is added into the grid with plain stores; items of a longer list add
+// with device atomics.
+__global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, const FacetDev *facets,
+                                                           const int *listFacet, const int2 *items,
+                                                           long long nitems, Common cm,
+                                                           const int *counts, const int *offsets,
+                                                           const int *lists) {
+  __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
+  __shared__ float4 s_stage[kWaves][kStageN * kStage / 4];
+  __shared__ int s_cell[kWaves][64];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const bool tap = lane < kW * kW;
+  // spare lanes read a real staged entry; their sums are never handed over
+  const int tjv = tap ? lane / kW : 0, tju = tap ? lane - kW * (lane / kW) : 0;
+  const int offU = tju, offV = 8 + 2 * tjv;
+  const int laneOff = tap ? tjv * kS + tju : lane;  // == lane (mod 64) on every lane
+  const int tjuB = tap ? tju : (1 << 20);           // spare lanes never land in the tile
+  float *re = reinterpret_cast<float *>(s_tile4[wv]);
+  float *im = re + kPlane;
+  float *stg = reinterpret_cast<float *>(s_stage[wv]);
+  int *scell = s_cell[wv];
+  const float4 *gcf8 = reinterpret_cast<const float4 *>(cm.gcf);
+  for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
+       w += (long long)gridDim.x * kWaves) {
     const int2 item = items[w];
     const long long L = item.x;
     const int n = counts[L];
@@ -324,65 +350,147 @@ __global__ __launch_bounds__(kThreads) void k_grid_chunks(const float *vis, cons
     const int t = (int)(rel - (long long)p * per);
     const int tv = t / f.ntu, tu = t - tv * f.ntu;
     const int col0 = tu * TU, row0 = tv * TV;
-    __syncthreads();
-    for (int i = threadIdx.x; i < TV * TU; i += kThreads) { s_re[i] = 0.0f; s_im[i] = 0.0f; }
-    __syncthreads();
+    for (int i = lane; i < 2 * kPlane / 4; i += 64) s_tile4[wv][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const int *lst = lists + offsets[L];
     const int npc = cm.pcount[p];
+    if (npc <= 0) continue;  // planes without channels have no tracks
     const int *pch = cm.pchan + (size_t)p * cm.maxpc;
-    for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
-      const long long r = lst[e];
-      const float *row = vis + r * (long long)cm.lenvis;
-      float uu, vv, ww;
-      rotate_uvw(f, row, uu, vv, ww);
-      float ar[kW * kW], ai[kW * kW];
+    int cur = kNoCell;               // packed base cell of the running sums
+    float ar = 0.0f, ai = 0.0f;      // running sums of this tap lane
+    int pend = kPark + lane;         // plane index of the deferred write
+    float pvr = 0.0f, pvi = 0.0f;    // plane values read for it
+    float par = 0.0f, pai = 0.0f;    // sums to add to them
+    auto handover = [&]() {
+      re[pend] = pvr + par;
+      im[pend] = pvi + pai;
+      // kNoCell decodes to a column far right of any tile, so it parks
+      const int cs = (cur & 0xffff) - col0, rs = (cur >> 16) - row0;
+      const int idx = rs * kS + cs + laneOff;
+      const bool in = (unsigned)(cs + tjuB) < (unsigned)TU && (unsigned)(rs + tjv) < (unsigned)TV;
+      pend = in ? idx : (kPark | (idx & 63));
+      pvr = re[pend];
+      pvi = im[pend];
+      par = ar;
+      pai = ai;
+    };
+    // slot (track e, channel index c) of this lane in the current pass and
+    // the next; a pass advances every lane by 64 slots
+    const int dq = 64 / npc, dr = 64 - dq * npc;
+    int ea = e0 + lane / npc, ca = lane - (lane / npc) * npc;
+    int eb = ea + dq, cb = ca + dr;
+    if (cb >= npc) { cb -= npc; ++eb; }
+    float xu = 0.0f, xv = 0.0f, xw = 0.0f, xre = 0.0f, xim = 0.0f, xwt = 0.0f;
+    int xch = 0;
+    if (ea < e1) {
+      const float *row = vis + (long long)lst[ea] * cm.lenvis;
+      xch = pch[ca];
+      xu = row[0]; xv = row[1]; xw = row[2];
+      const float *x = row + cm.nrparm + 3 * xch;
+      xre = x[0]; xim = x[1]; xwt = x[2];
+    }
+    long long rb = (eb < e1) ? lst[eb] : 0;
+    const int npass = ((e1 - e0) * npc + 63) / 64;
+    for (int pass = 0; pass < npass; ++pass) {
+      const bool live = ea < e1;
+      const float u0 = xu, v0 = xv, w0 = xw, re0 = xre, im0 = xim, wt0 = xwt;
+      const int ch0 = xch;
+      // fetch the next pass's rows and the index of the one after
+      int ec = eb + dq, cc2 = cb + dr;
+      if (cc2 >= npc) { cc2 -= npc; ++ec; }
+      if (eb < e1) {
+        const float *row = vis + rb * cm.lenvis;
+        xch = pch[cb];
+        xu = row[0]; xv = row[1]; xw = row[2];
+        const float *x = row + cm.nrparm + 3 * xch;
+        xre = x[0]; xim = x[1]; xwt = x[2];
+      }
+      const long long rc = (ec < e1) ? lst[ec] : 0;
+      bool keep = false;
+      Sample sm;
+      if (live) {
+        float uu, vv, ww;
+        rotate3(f, u0, v0, w0, uu, vv, ww);
+        sm = prep_vals(f, cm, uu, vv, ww, ch0, re0, im0, wt0);
+        keep = (sm.vr != 0.0f || sm.vi != 0.0f) && sm.iu + kW > col0 && sm.iu < col0 + TU &&
+               sm.iv + kW > row0 && sm.iv < row0 + TV;
+      }
+      ea = eb; ca = cb; eb = ec; cb = cc2; rb = rc;
+      const unsigned long long mask = __ballot(keep);
+      if (mask == 0) continue;
+      const int nv = __popcll(mask);
+      if (keep) {
+        const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
+        float4 *sp = reinterpret_cast<float4 *>(stg + pos * kStage);
+        const float4 g0 = gcf8[2 * sm.itu], g1 = gcf8[2 * sm.itu + 1];
+        const float4 h0 = gcf8[2 * sm.itv], h1 = gcf8[2 * sm.itv + 1];
+        sp[0] = g0;
+        sp[1] = g1;
+        sp[2] = make_float4(h0.x * sm.vr, h0.x * sm.vi, h0.y * sm.vr, h0.y * sm.vi);
+        sp[3] = make_float4(h0.z * sm.vr, h0.z * sm.vi, h0.w * sm.vr, h0.w * sm.vi);
+        sp[4] = make_float4(h1.x * sm.vr, h1.x * sm.vi, h1.y * sm.vr, h1.y * sm.vi);
+        sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, 0.0f, 0.0f);
+        scell[pos] = (sm.iu & 0xffff) | (sm.iv << 16);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int cellq = scell[lane];  // lane q: base cell of staged sample q
+      const int before = scell[lane > 0 ? lane - 1 : 0];
+      const int cprev = (lane == 0) ? cur : before;
+      const unsigned long long moves = __ballot(lane < nv && cellq != cprev);
+      for (int q = 0; q < nv; q += kGroup) {
+        float wu[kGroup];
+        float2 ab[kGroup];
 #pragma unroll
-      for (int j = 0; j < kW * kW; ++j) { ar[j] = 0.0f; ai[j] = 0.0f; }
-      int bu = 0, bv = 0;
-      bool have = false;
-      for (int k = 0; k < npc; ++k) {
-        const Sample s = prep_sample(f, cm, row, uu, vv, ww, pch[k]);
-        if (s.vr == 0.0f && s.vi == 0.0f) continue;
-        if (!have || s.iu != bu || s.iv != bv) {
-          if (have) flush_window(ar, ai, bu, bv, col0, row0, s_re, s_im);
-          bu = s.iu;
-          bv = s.iv;
-          have = true;
+        for (int j = 0; j < kGroup; ++j) {
+          wu[j] = stg[(q + j) * kStage + offU];
+          ab[j] = *reinterpret_cast<const float2 *>(stg + (q + j) * kStage + offV);
         }
-        const float4 u0 = s_gcf[2 * s.itu], u1 = s_gcf[2 * s.itu + 1];
-        const float4 v0 = s_gcf[2 * s.itv], v1 = s_gcf[2 * s.itv + 1];
-        const float cu[kW] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z};
-        const float cv[kW] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z};
 #pragma unroll
-        for (int jv = 0; jv < kW; ++jv) {
-          const float wr = s.vr * cv[jv], wi = s.vi * cv[jv];
-#pragma unroll
-          for (int ju = 0; ju < kW; ++ju) {
-            ar[jv * kW + ju] = fmaf(wr, cu[ju], ar[jv * kW + ju]);
-            ai[jv * kW + ju] = fmaf(wi, cu[ju], ai[jv * kW + ju]);
+        for (int j = 0; j < kGroup; ++j) {
+          if (q + j < nv) {
+            if ((moves >> (q + j)) & 1ull) {
+              handover();
+              cur = __builtin_amdgcn_readlane(cellq, q + j);
+              ar = 0.0f;
+              ai = 0.0f;
+            }
+            ar = fmaf(wu[j], ab[j].x, ar);
+            ai = fmaf(wu[j], ab[j].y, ai);
           }
         }
       }
-      if (have) flush_window(ar, ai, bu, bv, col0, row0, s_re, s_im);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __syncthreads();
+    handover();
+    re[pend] = pvr + par;
+    im[pend] = pvi + pai;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float *g = f.grid + (size_t)p * f.sizeGrid;
     const int ncol = f.lrow / 2;
-    for (int i = threadIdx.x; i < TV * TU; i += kThreads) {
+    for (int i = lane; i < TV * TU; i += 64) {
       const int rr = i / TU, cc = i - rr * TU;
       const int gr = row0 + rr, gc = col0 + cc;
       if (gr >= f.ny || gc >= ncol) continue;
-      const float re = s_re[i], im = s_im[i];
-      if (re == 0.0f && im == 0.0f) continue;
+      const float vr = re[rr * kS + cc], vi = im[rr * kS + cc];
+      if (vr == 0.0f && vi == 0.0f) continue;
       float *q = g + (size_t)gr * f.lrow + 2 * gc;
       if (multi) {
-        unsafeAtomicAdd(q, re);
-        unsafeAtomicAdd(q + 1, im);
+        unsafeAtomicAdd(q, vr);
+        unsafeAtomicAdd(q + 1, vi);
       } else {
-        q[0] += re;
-        q[1] += im;
+        q[0] += vr;
+        q[1] += vi;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -551,6 +659,10 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
     f.nx = fi->nx; f.ny = fi->ny; f.hw = W / 2;
     f.lrow = 2 * (1 + fi->nx / 2 + f.hw);
     if (fi->sizeGrid < f.lrow * fi->ny) return fail(kInvalid, "facet %d: sizeGrid too small", gs->facets[k]);
+    // base cells are packed as 16-bit column | 16-bit signed row
+    if (fi->nx < 2 || fi->ny < 2 || fi->nx > 65536 || fi->ny > 32768)
+      return fail(kInvalid, "facet %d: %d x %d grid outside 2..65536 x 2..32768", gs->facets[k], fi->nx,
+                  fi->ny);
     f.uscale = fi->uscale; f.vscale = fi->vscale;
     for (int j = 0; j < 3; ++j) f.shift[j] = fi->shift[j];
     for (int j = 0; j < 9; ++j) f.rot[j] = fi->rotUV[j];
@@ -604,7 +716,7 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   // (re)allocate device tables
   (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
   (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
-  OBIT_HIP_TRY(hipMalloc(&gs->d_gcf, sizeof(float) * kGcf));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_gcf, sizeof(float) * kGcf8));
   OBIT_HIP_TRY(hipMalloc(&gs->d_freq, sizeof(float) * nchan));
   OBIT_HIP_TRY(hipMalloc(&gs->d_sig, sizeof(float) * 3 * nchan));
   OBIT_HIP_TRY(hipMalloc(&gs->d_pf, sizeof(float) * 2 * nplane));
@@ -612,7 +724,11 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   OBIT_HIP_TRY(hipMalloc(&gs->d_pcount, sizeof(int) * nplane));
   OBIT_HIP_TRY(hipMalloc(&gs->d_facets, sizeof(FacetDev) * std::max(nf, 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_listFacet, sizeof(int) * std::max<long long>(gs->nlist, 1)));
-  OBIT_HIP_TRY(hipMemcpy(gs->d_gcf, gcf.data(), sizeof(float) * kGcf, hipMemcpyHostToDevice));
+  // rows of 8 (seven taps and a zero) so a row loads as two float4
+  std::vector<float> gcf8(kGcf8, 0.0f);
+  for (int it = 0; it <= N; ++it)
+    for (int c = 0; c < W; ++c) gcf8[8 * it + c] = gcf[(size_t)W * it + c];
+  OBIT_HIP_TRY(hipMemcpy(gs->d_gcf, gcf8.data(), sizeof(float) * kGcf8, hipMemcpyHostToDevice));
   OBIT_HIP_TRY(hipMemcpy(gs->d_freq, gi->h_freqArr, sizeof(float) * nchan, hipMemcpyHostToDevice));
   std::vector<float> sig(3 * (size_t)nchan, 0.0f);
   if (gi->h_sigma1) std::memcpy(sig.data(), gi->h_sigma1, sizeof(float) * nchan);
@@ -700,8 +816,8 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
   OBIT_HIP_TRY(hipGetLastError());
   if (nitems) {
-    const long long nbG = std::min<long long>(nitems, (long long)gs->nCU * 8);
-    hipLaunchKernelGGL(k_grid_chunks, dim3((unsigned)nbG), dim3(kThreads), 0, gs->stream, vis,
+    const long long nbG = std::min<long long>((nitems + kWaves - 1) / kWaves, (long long)gs->nCU * 8);
+    hipLaunchKernelGGL(k_grid_taps, dim3((unsigned)nbG), dim3(64 * kWaves), 0, gs->stream, vis,
                        gs->d_facets, gs->d_listFacet, (const int2 *)gs->d_items, nitems, gs->cm,
                        gs->d_counts, gs->d_offsets, gs->d_lists);
     OBIT_HIP_TRY(hipGetLastError());
@@ -989,6 +1105,25 @@ int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long
   OBIT_HIP_TRY(hipMemcpy(itv, di + 3 * ns, sizeof(int) * ns, hipMemcpyDeviceToHost));
   OBIT_HIP_TRY(hipMemcpy(vis, dvo, sizeof(float) * 2 * ns, hipMemcpyDeviceToHost));
   (void)hipFree(dv); (void)hipFree(dvo); (void)hipFree(di);
+  return obit::kOK;
+}
+
+int ObitHIPGridListStats(CUDAGridInfo *in, long iGPU, long long *stats) {
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  if (!gs || !stats) return obit::fail(obit::kState, "gridder not allocated");
+  long long nonEmpty = 0, entries = 0, maxc = 0, multi = 0;
+  for (int c : gs->h_counts) {
+    if (c > 0) ++nonEmpty;
+    entries += c;
+    if (c > maxc) maxc = c;
+    if (c > obit::grid::kChunk) ++multi;
+  }
+  stats[0] = gs->nlist;
+  stats[1] = nonEmpty;
+  stats[2] = entries;
+  stats[3] = (long long)gs->h_items.size();
+  stats[4] = maxc;
+  stats[5] = multi;
   return obit::kOK;
 }
 
