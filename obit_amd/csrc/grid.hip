@@ -33,7 +33,7 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 32;        // tile height in grid rows
+constexpr int TV = 30;        // tile height in grid rows
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
 
@@ -152,13 +152,14 @@ __device__ __forceinline__ void cells_at(const FacetDev &f, float uu, float vv, 
 }
 
 // fast_rot + fast_prep_grid for one (row, channel) of one facet, given
-// the channel's (re, im, weight).
+// the channel's frequency factor and (re, im, weight).
 __device__ __forceinline__ Sample prep_vals(const FacetDev &f, const Common &cm, float uu, float vv,
-                                            float ww, int ich, float vre, float vim, float vwt) {
+                                            float ww, int ich, float fsc, float vre, float vim,
+                                            float vwt) {
 #pragma clang fp contract(off)
   Sample s;
   const float phaseSign = (uu <= 0.0f) ? -1.0f : 1.0f;
-  const float freqFact = phaseSign * cm.freq[ich];
+  const float freqFact = phaseSign * fsc;
   float u = uu * freqFact, v = vv * freqFact, w = ww * freqFact;
   // fast_rot: phase shift (double phase, reduced, FP32 Cephes sincos)
   float vr, vi;
@@ -222,7 +223,7 @@ __device__ __forceinline__ Sample prep_vals(const FacetDev &f, const Common &cm,
 __device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *row, float uu,
                               float vv, float ww, int ich) {
   const float *vis = row + cm.nrparm + ich * 3;
-  return prep_vals(f, cm, uu, vv, ww, ich, vis[0], vis[1], vis[2]);
+  return prep_vals(f, cm, uu, vv, ww, ich, cm.freq[ich], vis[0], vis[1], vis[2]);
 }
 
 // ---------------- binning ----------------
@@ -287,7 +288,8 @@ constexpr int kPark = ((TV * kS + 63) / 64) * 64;  // 64 parking slots after the
 constexpr int kPlane = kPark + 64;                 // floats per re / im plane
 constexpr int kStage = 24;     // floats per staged sample: 8 u-weights, 8 (re, im) v-products
 constexpr int kGroup = 4;      // staged samples read ahead per step
-constexpr int kStageN = 64 + kGroup;  // staged samples per wave, with read-ahead slack
+constexpr int kStageN = 64 + 2 * kGroup;  // staged samples per wave, with read-ahead slack
+static_assert(64 % (2 * kGroup) == 0, "the walk reads at most 2 groups past the last sample");
 constexpr int kNoCell = 0x7fffffff;
 static_assert(kS % 64 == 7 && kS >= TU, "tile stride must be 7 mod 64");
 static_assert(kPlane % 4 == 0, "planes are cleared as float4");
@@ -322,6 +324,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
   __shared__ float4 s_stage[kWaves][kStageN * kStage / 4];
   __shared__ int s_cell[kWaves][64];
+  __shared__ float4 s_gcf8[101 * 2];  // row it: taps 0..6 of fractional offset it, then 0
+  for (int i = threadIdx.x; i < 101 * 2; i += blockDim.x)
+    s_gcf8[i] = reinterpret_cast<const float4 *>(cm.gcf)[i];
+  __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const bool tap = lane < kW * kW;
   // spare lanes read a real staged entry; their sums are never handed over
@@ -333,7 +339,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
   float *im = re + kPlane;
   float *stg = reinterpret_cast<float *>(s_stage[wv]);
   int *scell = s_cell[wv];
-  const float4 *gcf8 = reinterpret_cast<const float4 *>(cm.gcf);
   for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
        w += (long long)gridDim.x * kWaves) {
     const int2 item = items[w];
@@ -379,11 +384,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     int ea = e0 + lane / npc, ca = lane - (lane / npc) * npc;
     int eb = ea + dq, cb = ca + dr;
     if (cb >= npc) { cb -= npc; ++eb; }
-    float xu = 0.0f, xv = 0.0f, xw = 0.0f, xre = 0.0f, xim = 0.0f, xwt = 0.0f;
+    float xu = 0.0f, xv = 0.0f, xw = 0.0f, xre = 0.0f, xim = 0.0f, xwt = 0.0f, xfs = 0.0f;
     int xch = 0;
     if (ea < e1) {
       const float *row = vis + (long long)lst[ea] * cm.lenvis;
       xch = pch[ca];
+      xfs = cm.freq[xch];
       xu = row[0]; xv = row[1]; xw = row[2];
       const float *x = row + cm.nrparm + 3 * xch;
       xre = x[0]; xim = x[1]; xwt = x[2];
@@ -392,7 +398,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     const int npass = ((e1 - e0) * npc + 63) / 64;
     for (int pass = 0; pass < npass; ++pass) {
       const bool live = ea < e1;
-      const float u0 = xu, v0 = xv, w0 = xw, re0 = xre, im0 = xim, wt0 = xwt;
+      const float u0 = xu, v0 = xv, w0 = xw, re0 = xre, im0 = xim, wt0 = xwt, fs0 = xfs;
       const int ch0 = xch;
       // fetch the next pass's rows and the index of the one after
       int ec = eb + dq, cc2 = cb + dr;
@@ -400,6 +406,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       if (eb < e1) {
         const float *row = vis + rb * cm.lenvis;
         xch = pch[cb];
+        xfs = cm.freq[xch];
         xu = row[0]; xv = row[1]; xw = row[2];
         const float *x = row + cm.nrparm + 3 * xch;
         xre = x[0]; xim = x[1]; xwt = x[2];
@@ -410,7 +417,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       if (live) {
         float uu, vv, ww;
         rotate3(f, u0, v0, w0, uu, vv, ww);
-        sm = prep_vals(f, cm, uu, vv, ww, ch0, re0, im0, wt0);
+        sm = prep_vals(f, cm, uu, vv, ww, ch0, fs0, re0, im0, wt0);
         keep = (sm.vr != 0.0f || sm.vi != 0.0f) && sm.iu + kW > col0 && sm.iu < col0 + TU &&
                sm.iv + kW > row0 && sm.iv < row0 + TV;
       }
@@ -422,8 +429,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
         const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
         float4 *sp = reinterpret_cast<float4 *>(stg + pos * kStage);
-        const float4 g0 = gcf8[2 * sm.itu], g1 = gcf8[2 * sm.itu + 1];
-        const float4 h0 = gcf8[2 * sm.itv], h1 = gcf8[2 * sm.itv + 1];
+        const float4 g0 = s_gcf8[2 * sm.itu], g1 = s_gcf8[2 * sm.itu + 1];
+        const float4 h0 = s_gcf8[2 * sm.itv], h1 = s_gcf8[2 * sm.itv + 1];
         sp[0] = g0;
         sp[1] = g1;
         sp[2] = make_float4(h0.x * sm.vr, h0.x * sm.vi, h0.y * sm.vr, h0.y * sm.vi);
@@ -432,6 +439,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
         sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, 0.0f, 0.0f);
         scell[pos] = (sm.iu & 0xffff) | (sm.iv << 16);
       }
+      if (lane < kGroup) {
+        float4 *sp = reinterpret_cast<float4 *>(stg + (nv + lane) * kStage);
+#pragma unroll
+        for (int k = 0; k < kStage / 4; ++k) sp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -439,27 +451,39 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       const int before = scell[lane > 0 ? lane - 1 : 0];
       const int cprev = (lane == 0) ? cur : before;
       const unsigned long long moves = __ballot(lane < nv && cellq != cprev);
-      for (int q = 0; q < nv; q += kGroup) {
-        float wu[kGroup];
-        float2 ab[kGroup];
+      // Walk the staged samples kGroup at a time, reading the next group
+      // while the current one accumulates (two register sets in turn); the
+      // zeroed entries after the last sample make a partial group harmless.
+      float wa[kGroup], wb[kGroup];
+      float2 aa[kGroup], ab[kGroup];
+      auto load = [&](float *w, float2 *a, int q0) {
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) {
-          wu[j] = stg[(q + j) * kStage + offU];
-          ab[j] = *reinterpret_cast<const float2 *>(stg + (q + j) * kStage + offV);
+          w[j] = stg[(q0 + j) * kStage + offU];
+          a[j] = *reinterpret_cast<const float2 *>(stg + (q0 + j) * kStage + offV);
         }
+      };
+      auto walk = [&](const float *w, const float2 *a, int q0) {
+        const unsigned long long mv = moves >> q0;
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) {
-          if (q + j < nv) {
-            if ((moves >> (q + j)) & 1ull) {
-              handover();
-              cur = __builtin_amdgcn_readlane(cellq, q + j);
-              ar = 0.0f;
-              ai = 0.0f;
-            }
-            ar = fmaf(wu[j], ab[j].x, ar);
-            ai = fmaf(wu[j], ab[j].y, ai);
+          if (mv & (1ull << j)) {
+            handover();
+            cur = __builtin_amdgcn_readlane(cellq, q0 + j);
+            ar = 0.0f;
+            ai = 0.0f;
           }
+          ar = fmaf(w[j], a[j].x, ar);
+          ai = fmaf(w[j], a[j].y, ai);
         }
+      };
+      load(wa, aa, 0);
+      for (int q = 0; q < nv; q += 2 * kGroup) {
+        load(wb, ab, q + kGroup);
+        walk(wa, aa, q);
+        if (q + kGroup >= nv) break;
+        load(wa, aa, q + 2 * kGroup);
+        walk(wb, ab, q + kGroup);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
