@@ -40,7 +40,7 @@ def _events(lib):
     return lib.ObitCUDAEventCreate(), lib.ObitCUDAEventCreate()
 
 
-def bench_dft(lib, steps, warmup, sample_rows):
+def bench_dft(lib, steps, warmup, sample_rows, variant=None, accuracy=True):
     from obit_amd import synth
     from obit_amd.skymodel import SkyModelDFT
     t0 = time.time()
@@ -54,8 +54,13 @@ def bench_dft(lib, steps, warmup, sample_rows):
     try:
         sm.set_vis(uv.buffer)
         t0 = time.time()
-        res["variant"] = dict(zip(("cpt", "tile", "wg_per_cu"), sm.autotune(sample_rows=32768, reps=1)))
-        log(f"autotune -> {res['variant']} in {time.time() - t0:.1f}s")
+        if variant:
+            sm.set_variant(*variant)
+            res["variant"] = dict(zip(("cpt", "tile", "wg_per_cu"), sm.variant()))
+            log(f"variant {res['variant']} (fixed)")
+        else:
+            res["variant"] = dict(zip(("cpt", "tile", "wg_per_cu"), sm.autotune(sample_rows=32768, reps=1)))
+            log(f"autotune -> {res['variant']} in {time.time() - t0:.1f}s")
         st = sm.stream()
         for _ in range(warmup):
             sm.graph_launch()
@@ -77,6 +82,8 @@ def bench_dft(lib, steps, warmup, sample_rows):
         log(f"split DFT: {kms:.2f} ms/launch, {res['kernel_evals_per_s'] / 1e12:.3f} Tevals/s")
     finally:
         sm.close()
+    if not accuracy:
+        return res, uv, sky
 
     # accuracy and the FP64 comparator on a row sample
     rng = np.random.default_rng(5)
@@ -146,6 +153,7 @@ def bench_grid(lib, steps):
         lib.ObitHIPFree(dptr)
     nf = len(facets)
     samples = uv.nrow * nci * nf
+    flop = 245.0 * samples  # 49 weight products + 98 complex-MAC FMAs per splat
     lrec_bytes = uv.lrec * 4
     grid_bytes = nf * 8 * facets[0].ny * (facets[0].nx // 2 + 1) * 8
     stream_bytes = nf * uv.nrow * lrec_bytes + grid_bytes
@@ -156,10 +164,51 @@ def bench_grid(lib, steps):
         "rows_chan_per_s": uv.nrow * nci / (ms * 1e-3),
         "per_facet_stream_GBps": stream_bytes / (ms * 1e-3) / 1e9,
         "per_facet_stream_frac_hbm": stream_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {
+            "bound": "hbm", "achieved": stream_bytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": stream_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": read_traffic("grid_traffic.json"),
+            "bytes_model": "per-facet stream: facets x rows x record bytes + grids",
+            "valu_TFLOPs": flop / (ms * 1e-3) / 1e12,
+            "valu_frac": flop / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+        },
     })
     log(f"gridding: {ms:.2f} ms/pass, {res['samples_per_s'] / 1e9:.2f} Gsamples/s, "
         f"{res['per_facet_stream_GBps']:.0f} GB/s per-facet-stream model")
     return res
+
+
+def cpu_grid_baseline(seconds):
+    """Obit's threaded CPU gridder (restated in C: fast_prep_grid + 7x7
+    scatter, rows split over threads into nGpI private grids, merged) on
+    whole facets of the wideband workload, all 256 channels into one
+    plane, facet after facet until about `seconds` have passed."""
+    from obit_amd import synth
+    from oracle import oracle as orc
+    cores = _cores()
+    uv, _, facets = synth.workload(4)
+    ngpi = min(cores, 16)
+    t0 = time.perf_counter()
+    done = 0
+    for f in facets:
+        orc.grid_obit_threaded(uv, f, nGpI=ngpi, nthreads=cores)
+        done += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    n = done * uv.nrow * uv.nchan * uv.nif
+    log(f"cpu gridding: {done} facets x {uv.nrow} rows x 256 ch in {dt:.1f}s on {cores} threads "
+        f"-> {n / dt / 1e9:.3f} Gsamples/s")
+    return {"value": n / dt, "unit": "samples/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of the 40 facets, every row and channel ({n:.3g} samples, {dt:.1f} s)"}
+
+
+def _cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 64))
 
 
 def cpu_baseline(uv, sky, seconds):
@@ -167,11 +216,7 @@ def cpu_baseline(uv, sky, seconds):
     sincos, FP64 sums, ObitSkyModelFTDFT row split) on a row sample sized
     for about `seconds` of CPU time, on this host's cores."""
     from oracle import oracle as orc
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 64))
+    cores = _cores()
     rng = np.random.default_rng(9)
     n = 64
     per_comp = uv.nif * uv.nchan * sky.comps.shape[0]
@@ -192,8 +237,29 @@ def cpu_baseline(uv, sky, seconds):
                       f"10,000 components ({n * per_comp:.3g} evals, {dt:.1f} s)"}
 
 
-def read_traffic():
-    p = os.path.join(ROOT, "profiles", "dft_traffic.json")
+def copy_calibration(lib, mib):
+    """Three launches of the library's 16-byte copy kernel over `mib` MiB
+    (reads and writes exactly 2 x mib MiB each): a profile of this run
+    gives the byte counters' reading for a known amount of traffic."""
+    from obit_amd import _lib
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(a), n), "malloc")
+    _lib.check(lib.ObitHIPMalloc(C.byref(b), n), "malloc")
+    try:
+        for _ in range(3):
+            _lib.check(lib.ObitHIPCopyKernel(b, a, n, None), "copy")
+        _lib.check(lib.ObitHIPDeviceSynchronize(), "sync")
+        log(f"copy kernel: 3 x {mib} MiB")
+    finally:
+        lib.ObitHIPFree(a)
+        lib.ObitHIPFree(b)
+
+
+def read_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factors)."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
@@ -206,7 +272,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--sample-rows", type=int, default=256)
+    ap.add_argument("--sample-rows", type=int, default=4096,
+                    help="rows of the FP64 comparator / accuracy sample")
+    ap.add_argument("--variant", type=str, default=None,
+                    help="cpt,tile,wg_per_cu: fixed DFT variant instead of the autotuner")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 comparator")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run the copy kernel over this many MiB three times "
+                         "(calibrates byte counters)")
     ap.add_argument("--grid-steps", type=int, default=3)
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -223,12 +296,28 @@ def main():
     _lib.check(lib.ObitHIPDeviceInfo(0, name, 64, C.byref(ncu), C.byref(clk), C.byref(mem)), "device")
     log(f"device {name.value.decode()} {ncu.value} CUs {clk.value / 1e3:.0f} MHz {mem.value / 2**30:.0f} GiB")
 
+    if args.copy_mib:
+        copy_calibration(lib, args.copy_mib)
     if args.grid_only:
         print(json.dumps({"gridding": bench_grid(lib, args.grid_steps)}), flush=True)
         return
-    dft, uv, sky = bench_dft(lib, args.steps, args.warmup, args.sample_rows)
+    variant = tuple(int(x) for x in args.variant.split(",")) if args.variant else None
+    dft, uv, sky = bench_dft(lib, args.steps, args.warmup, args.sample_rows, variant,
+                             accuracy=not args.no_accuracy)
     grid = None if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)
+    if grid is not None and not args.no_cpu:
+        grid["cpu_baseline"] = cpu_grid_baseline(args.cpu_seconds / 2)
+    if not args.no_accuracy:
+        dft_extra = {
+            "err_max_vs_fp64_rel_peak": dft["err_max_rel_peak"],
+            "err_rms_vs_fp64_rel_peak": dft["err_rms_rel_peak"],
+            "fp64_gpu_evals_per_s": dft["fp64_evals_per_s"],
+            "split_vs_fp64_gpu": dft["speedup_vs_fp64_gpu"],
+            "accuracy_rows": args.sample_rows,
+        }
+    else:
+        dft_extra = {}
 
     achieved = FLOP_PER_EVAL * dft["evals_per_step"] / (dft["kernel_ms"] * 1e-3) / 1e12
     line = {
@@ -250,10 +339,7 @@ def main():
             "rows": dft["rows"], "channels": dft["channels"], "components": dft["components"],
             "precision": "FP64 phase + range reduction, FP32 sincos/recurrence/sums, FP64 tile folds",
             "variant": dft["variant"],
-            "err_max_vs_fp64_rel_peak": dft["err_max_rel_peak"],
-            "err_rms_vs_fp64_rel_peak": dft["err_rms_rel_peak"],
-            "fp64_gpu_evals_per_s": dft["fp64_evals_per_s"],
-            "split_vs_fp64_gpu": dft["speedup_vs_fp64_gpu"],
+            **dft_extra,
             "kernel_ms": dft["kernel_ms"],
         },
         "roofline": {
@@ -262,7 +348,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": read_traffic(),
+            "traffic": read_traffic("dft_traffic.json"),
         },
         "cpu_baseline": cpu,
         "gridding": grid,

@@ -587,6 +587,7 @@ struct GpuState {
   int device = 0;
   hipStream_t stream = nullptr;
   int nCU = 256;
+  int tapsPerCU = 0;        // resident k_grid_taps blocks per CU (0: not queried yet)
   std::vector<int> facets;  // facet indices on this GPU
   FacetDev *d_facets = nullptr;
   std::vector<FacetDev> h_facets;
@@ -840,7 +841,15 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
   OBIT_HIP_TRY(hipGetLastError());
   if (nitems) {
-    const long long nbG = std::min<long long>((nitems + kWaves - 1) / kWaves, (long long)gs->nCU * 8);
+    // persistent grid of exactly the resident blocks: more would queue a
+    // second round of blocks behind the first and idle CUs at the end
+    if (gs->tapsPerCU <= 0) {
+      int nb = 0;
+      OBIT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_grid_taps, 64 * kWaves, 0));
+      gs->tapsPerCU = std::max(nb, 1);
+    }
+    const long long nbG =
+        std::min<long long>((nitems + kWaves - 1) / kWaves, (long long)gs->nCU * gs->tapsPerCU);
     hipLaunchKernelGGL(k_grid_taps, dim3((unsigned)nbG), dim3(64 * kWaves), 0, gs->stream, vis,
                        gs->d_facets, gs->d_listFacet, (const int2 *)gs->d_items, nitems, gs->cm,
                        gs->d_counts, gs->d_offsets, gs->d_lists);
