@@ -119,6 +119,27 @@ def bench_dft(lib, steps, warmup, sample_rows, variant=None, accuracy=True):
     return res, uv, sky
 
 
+def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
+    """Obit's per-buffer path: ObitCUDASkyModelDFTCalc on host buffers of
+    nVisPIO rows (copy in to pinned memory, H2D, kernel, D2H, copy out),
+    over the first `rows` rows: the rate a caller that hands over host
+    buffers sees, PCIe included."""
+    from obit_amd.skymodel import SkyModelDFT
+    sub = uv.subset(np.arange(min(rows, uv.nrow)))
+    with SkyModelDFT(sub, sky, op="sub", nVisPIO=nvispio) as sm:
+        if variant:
+            sm.set_variant(*variant)
+        sm.calc(sub.buffer[:nvispio].copy())  # warm
+        buf = sub.buffer.copy()
+        t0 = time.perf_counter()
+        sm.calc(buf)
+        dt = time.perf_counter() - t0
+    ev = sub.nrow * sub.nif * sub.nchan * sky.comps.shape[0]
+    log(f"per-buffer path (host buffers, PCIe included): {sub.nrow} rows in {dt:.3f}s -> "
+        f"{ev / dt / 1e12:.3f} Tevals/s")
+    return {"pcie_inclusive_evals_per_s": ev / dt, "pcie_rows": sub.nrow, "nVisPIO": nvispio}
+
+
 def bench_grid(lib, steps):
     from obit_amd import synth, _lib
     from obit_amd.gridder import GPUGrid, plane_of_channel
@@ -304,6 +325,8 @@ def main():
     variant = tuple(int(x) for x in args.variant.split(",")) if args.variant else None
     dft, uv, sky = bench_dft(lib, args.steps, args.warmup, args.sample_rows, variant,
                              accuracy=not args.no_accuracy)
+    vt = tuple(dft["variant"][k] for k in ("cpt", "tile", "wg_per_cu"))
+    pcie = None if args.no_accuracy else bench_pcie(lib, uv, sky, vt)
     grid = None if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)
     if grid is not None and not args.no_cpu:
@@ -340,6 +363,7 @@ def main():
             "precision": "FP64 phase + range reduction, FP32 sincos/recurrence/sums, FP64 tile folds",
             "variant": dft["variant"],
             **dft_extra,
+            **(pcie or {}),
             "kernel_ms": dft["kernel_ms"],
         },
         "roofline": {

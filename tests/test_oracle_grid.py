@@ -121,3 +121,53 @@ def test_fp32_gridder_vs_fp64_and_threaded_merge(oracle):
     thr = oracle.grid_obit_threaded(uv, f, nGpI=4, nthreads=4)
     assert np.abs(thr - g64).max() < 2e-6 * peak
     assert np.abs(g64).max() > 0
+
+
+def test_flip_closed_forms(oracle):
+    """ThreadFlip (src/ObitThreadGrid.c:998-1045) folds the footprint cells
+    left of u = 0 onto their Hermitian mirror.  Pinned by two properties of
+    any such fold: the u = 0 column of the result is Hermitian in v, and for
+    one sample straddling u = 0 the half-plane sum follows from its GCF taps
+    alone (a real visibility counts the u = 0 column twice, an imaginary one
+    cancels it there and subtracts the cells left of it)."""
+    f = _facet(nx=64)
+    _, cg = oracle.convfunc()
+    lrow = 2 * (1 + f.nx // 2 + 3)
+    ncol = lrow // 2
+    hw = 3
+    for re, im in ((1.0, 0.0), (0.0, 1.0)):
+        uv = _one_row_uv(1.3 / (f.nx * float(np.float32(np.pi / 180 * float(np.float32(f.cell_deg))))),
+                         -7.6 / (f.nx * float(np.float32(np.pi / 180 * float(np.float32(f.cell_deg))))),
+                         re=re, im=im)
+        iu, iv, itu, itv, vis = oracle.grid_prep(uv, f)
+        cu = cg[7 * itu[0]:7 * itu[0] + 7].astype(np.float64)
+        cv = cg[7 * itv[0]:7 * itv[0] + 7].astype(np.float64)
+        cols = iu[0] + np.arange(7) - hw        # u of each tap column
+        assert cols.min() < 0 <= cols.max()    # straddles u = 0
+        g = oracle.grid_fp64(uv, f, flip=True)[:lrow * f.ny].reshape(f.ny, ncol, 2)
+        half = g[:, hw:, 0] + 1j * g[:, hw:, 1]  # u >= 0
+        s = half.sum()
+        tot_v = cv.sum()
+        if re:
+            expect = (cu.sum() + cu[cols == 0].sum()) * tot_v
+            assert abs(s.real - expect) < 1e-9 * expect and abs(s.imag) < 1e-12
+        else:
+            expect = (cu[cols > 0].sum() - cu[cols < 0].sum()) * tot_v
+            assert abs(s.imag - expect) < 1e-9 * abs(cu).sum() and abs(s.real) < 1e-12
+    # u = 0 column Hermitian in v after the flip, for random samples near u = 0
+    f2 = _facet(nx=128)
+    scale = f2.nx * float(np.float32(np.pi / 180 * float(np.float32(f2.cell_deg))))
+    rng = np.random.default_rng(3)
+    n = 60
+    hdr = np.stack([rng.uniform(-4, 4, n) / scale, rng.uniform(-40, 40, n) / scale,
+                    np.zeros(n), np.full(n, 258.0), np.zeros(n)], axis=1)
+    fa = synth.freq_table(1.4e9, 1, 1, 1e6)
+    uv = synth.make_uv(hdr, 1, 1, 1, 1.4e9, fa, rng, crval_stokes=1.0)
+    uv.buffer[:, 7::3] = 1.0
+    lrow2 = 2 * (1 + f2.nx // 2 + 3)
+    g = oracle.grid_fp64(uv, f2, flip=True)[:lrow2 * f2.ny].reshape(f2.ny, lrow2 // 2, 2)
+    c0 = g[:, hw, 0] + 1j * g[:, hw, 1]        # rows 0..ny-1, v = row - ny/2
+    ny = f2.ny
+    a = c0[1:]                                 # v = -ny/2+1 .. ny/2-1
+    assert np.abs(c0).max() > 0
+    assert np.allclose(a, np.conj(a[::-1]), rtol=0, atol=1e-12 * np.abs(c0).max())
