@@ -8,8 +8,10 @@
  * u <= 0; baseline limits on u^2+v^2; guard band tested in wavelengths;
  * GCF taps selected with the CPU's index arithmetic, so cell indices and
  * taps are identical to the CPU's; flip folds rows 1..ny/2.  Accumulation
- * is in FP32 in LDS tiles; each grid cell is written by exactly one
- * workgroup, with no global atomics.
+ * is in FP32 in per-wave LDS tiles without atomics; a tile is added into
+ * the grid once per work item, with plain stores when one item covers the
+ * tile's whole track list and with device atomics when a long list is
+ * split over several items.
  * Entry points abort with a message on a HIP error, as the reference does.
  */
 #ifndef OBITCUDAGRID_H
