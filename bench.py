@@ -164,9 +164,10 @@ def bench_grid(lib, steps):
             lib.ObitCUDAEventRecord(e1, st)
             lib.ObitHIPDeviceSynchronize()
             ms = lib.ObitHIPEventElapsedMs(e0, e1) / steps
-            st6 = (C.c_longlong * 6)()
-            _lib.check(lib.ObitHIPGridListStats(gg.c, 0, st6), "list stats")
-            res["lists"] = dict(zip(("lists", "non_empty", "entries", "items", "longest", "split"), list(st6)))
+            st8 = (C.c_longlong * 8)()
+            _lib.check(lib.ObitHIPGridListStats(gg.c, 0, st8), "list stats")
+            res["lists"] = dict(zip(("lists", "non_empty", "entries", "items", "longest", "split",
+                                     "tile_samples", "cell_moves"), list(st8)))
             log(f"gridding lists: {res['lists']}")
         finally:
             gg.close()

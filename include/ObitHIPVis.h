@@ -98,9 +98,11 @@ int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out);
  * (first of the 7 rows), itu, itv (GCF block 0..100) and vis[2*(..)]. */
 int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis,
                          int *iu, int *iv, int *itu, int *itv, float *vis);
-/* Statistics of the last binning pass on iGPU: stats[0] tile lists,
- * [1] non-empty lists, [2] list entries (track x tile), [3] work items,
- * [4] longest list, [5] lists split into several work items. */
+/* Statistics of the last gridding pass on iGPU (stats has 8 entries):
+ * [0] tile lists, [1] non-empty lists, [2] list entries (track x tile),
+ * [3] work items, [4] longest list, [5] lists split into several work
+ * items, [6] samples accumulated into tiles (sample x tile), [7] moves of
+ * the accumulating base cell among them (tile read-modify-writes). */
 int ObitHIPGridListStats(CUDAGridInfo *in, long iGPU, long long *stats);
 void *ObitHIPGridStream(CUDAGridInfo *in, long iGPU);
 

@@ -13,10 +13,12 @@
 // A binning pass lists, for each tile, the (row, plane) tracks whose 7x7
 // footprints can touch it (the footprint of a track is bounded by the
 // samples at the plane's lowest and highest frequency, since cells are
-// monotonic in frequency).  One workgroup per non-empty tile accumulates
-// its tracks in LDS with LDS float atomics and then adds the tile to the
-// grid in HBM with plain loads/stores: no cell is written by two
-// workgroups, so there are no global atomics.
+// monotonic in frequency).  Lists are cut into work items of at most
+// kChunk tracks; a wave takes an item, accumulates it in a private LDS
+// tile with lanes = the 49 footprint taps (no atomics, see k_grid_taps)
+// and adds the tile into the grid in HBM: with plain loads/stores when the
+// item is the tile's whole list, with device atomics when a long list was
+// split over several items.
 #include "common.h"
 #include "ObitCUDAGrid.h"
 #include "ObitCUDAUtil.h"
@@ -33,7 +35,7 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 30;        // tile height in grid rows
+constexpr int TV = 16;        // tile height in grid rows (LDS for 4 two-wave blocks per CU)
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
 
@@ -320,7 +322,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
                                                            const int *listFacet, const int2 *items,
                                                            long long nitems, Common cm,
                                                            const int *counts, const int *offsets,
-                                                           const int *lists) {
+                                                           const int *lists,
+                                                           unsigned long long *walk) {
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
   __shared__ float4 s_stage[kWaves][kStageN * kStage / 4];
   __shared__ int s_cell[kWaves][64];
@@ -360,6 +363,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     const int npc = cm.pcount[p];
     if (npc <= 0) continue;  // planes without channels have no tracks
     const int *pch = cm.pchan + (size_t)p * cm.maxpc;
+    unsigned nStaged = 0, nMoves = 0;  // walk statistics of this item
     int cur = kNoCell;               // packed base cell of the running sums
     float ar = 0.0f, ai = 0.0f;      // running sums of this tap lane
     int pend = kPark + lane;         // plane index of the deferred write
@@ -451,6 +455,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       const int before = scell[lane > 0 ? lane - 1 : 0];
       const int cprev = (lane == 0) ? cur : before;
       const unsigned long long moves = __ballot(lane < nv && cellq != cprev);
+      nStaged += nv;
+      nMoves += __popcll(moves);
       // Walk the staged samples kGroup at a time, reading the next group
       // while the current one accumulates (two register sets in turn); the
       // zeroed entries after the last sample make a partial group harmless.
@@ -492,6 +498,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     handover();
     re[pend] = pvr + par;
     im[pend] = pvi + pai;
+    if (walk && lane == 0) {
+      atomicAdd(&walk[0], (unsigned long long)nStaged);
+      atomicAdd(&walk[1], (unsigned long long)nMoves);
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -597,6 +607,7 @@ struct GpuState {
   long long listCap = 0;
   int2 *d_items = nullptr;
   long long itemCap = 0;
+  unsigned long long *d_walk = nullptr;  // staged samples, base-cell moves of the last pass
   std::vector<int> h_counts, h_offsets;
   std::vector<int2> h_items;
   // tables
@@ -797,6 +808,10 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
     hipLaunchKernelGGL(k_zero2, dim3(nbz), dim3(256), 0, gs->stream, gs->d_counts, gs->d_cursor, nl + 1);
     OBIT_HIP_TRY(hipGetLastError());
   }
+  if (!gs->d_walk) OBIT_HIP_TRY(hipMalloc(&gs->d_walk, 2 * sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_walk,
+                     (int *)gs->d_walk + 2, 2LL);
+  OBIT_HIP_TRY(hipGetLastError());
   const long long ntr = nrow * (long long)gs->cm.nplane * nf;
   const int nbB = (int)std::min<long long>((ntr + 255) / 256, (long long)gs->nCU * 32);
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
@@ -852,7 +867,7 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
         std::min<long long>((nitems + kWaves - 1) / kWaves, (long long)gs->nCU * gs->tapsPerCU);
     hipLaunchKernelGGL(k_grid_taps, dim3((unsigned)nbG), dim3(64 * kWaves), 0, gs->stream, vis,
                        gs->d_facets, gs->d_listFacet, (const int2 *)gs->d_items, nitems, gs->cm,
-                       gs->d_counts, gs->d_offsets, gs->d_lists);
+                       gs->d_counts, gs->d_offsets, gs->d_lists, gs->d_walk);
     OBIT_HIP_TRY(hipGetLastError());
   }
   // the host vectors are read by the async copies above
@@ -1024,7 +1039,7 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
     (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
-    (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis);
+    (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis); (void)hipFree(gs->d_walk);
     if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
     (void)hipStreamDestroy(gs->stream);
     obit::grid::GridCtx *c = obit::grid::get_ctx(cudaInfo, false);
@@ -1157,6 +1172,12 @@ int ObitHIPGridListStats(CUDAGridInfo *in, long iGPU, long long *stats) {
   stats[3] = (long long)gs->h_items.size();
   stats[4] = maxc;
   stats[5] = multi;
+  unsigned long long walk[2] = {0, 0};
+  if (gs->d_walk) {
+    OBIT_HIP_TRY(hipMemcpy(walk, gs->d_walk, sizeof(walk), hipMemcpyDeviceToHost));
+  }
+  stats[6] = (long long)walk[0];
+  stats[7] = (long long)walk[1];
   return obit::kOK;
 }
 
