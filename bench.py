@@ -140,6 +140,35 @@ def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
     return {"pcie_inclusive_evals_per_s": ev / dt, "pcie_rows": sub.nrow, "nVisPIO": nvispio}
 
 
+def grid_accuracy(uv, facets, b, e, rows=2000, nfacet=2):
+    """Gridded planes of a seeded row sample (2 facets x 8 planes) against
+    the oracle's FP64 gridder on the same samples: max and RMS |dG| over
+    the peak of each plane, worst plane reported."""
+    from obit_amd.gridder import GPUGrid, plane_of_channel
+    from oracle import oracle as orc
+    rng = np.random.default_rng(17)
+    sub = uv.subset(np.sort(rng.choice(uv.nrow, rows, replace=False)))
+    nci = uv.nchan * uv.nif
+    poc = plane_of_channel(nci, b, e, cpu_parity=False)
+    fs = facets[:nfacet]
+    emax = erms = 0.0
+    with GPUGrid(sub, fs, nplane=len(b), plane_of_chan=poc) as gg:
+        gg.grid(sub.buffer)
+        gg.flip()
+        for i, f in enumerate(fs):
+            g = gg.to_host(i)
+            for p in range(len(b)):
+                ref = orc.trim_centre(orc.grid_fp64(sub, f, bChan=int(b[p]), eChan=int(e[p]) + 1), f)
+                pk = np.abs(ref).max()
+                d = np.abs(g[p] - ref)
+                emax = max(emax, float(d.max() / pk))
+                erms = max(erms, float(np.sqrt(np.mean(d[ref != 0] ** 2)) / pk))
+    log(f"gridding accuracy ({rows} rows, {nfacet} facets x {len(b)} planes): max |dG|/peak {emax:.2e}, "
+        f"rms {erms:.2e}")
+    return {"rows": rows, "facets": nfacet, "planes": len(b), "err_max_vs_fp64_rel_peak": emax,
+            "err_rms_vs_fp64_rel_peak": erms}
+
+
 def bench_grid(lib, steps):
     from obit_amd import synth, _lib
     from obit_amd.gridder import GPUGrid, plane_of_channel
@@ -173,6 +202,7 @@ def bench_grid(lib, steps):
             gg.close()
     finally:
         lib.ObitHIPFree(dptr)
+    res["accuracy"] = grid_accuracy(uv, facets, b, e)
     nf = len(facets)
     samples = uv.nrow * nci * nf
     flop = 245.0 * samples  # 49 weight products + 98 complex-MAC FMAs per splat
