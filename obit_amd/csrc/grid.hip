@@ -290,8 +290,9 @@ constexpr int kPark = ((TV * kS + 63) / 64) * 64;  // 64 parking slots after the
 constexpr int kPlane = kPark + 64;                 // floats per re / im plane
 constexpr int kStage = 24;     // floats per staged sample: 8 u-weights, 8 (re, im) v-products
 constexpr int kGroup = 4;      // staged samples read ahead per step
-constexpr int kStageN = 64 + 2 * kGroup;  // staged samples per wave, with read-ahead slack
-static_assert(64 % (2 * kGroup) == 0, "the walk reads at most 2 groups past the last sample");
+constexpr int kHalf = 32;      // kept samples staged at a time
+constexpr int kStageN = kHalf + 2 * kGroup;  // stage entries per wave, with read-ahead slack
+static_assert(kHalf % (2 * kGroup) == 0, "the walk reads at most 2 groups past the last sample");
 constexpr int kNoCell = 0x7fffffff;
 static_assert(kS % 64 == 7 && kS >= TU, "tile stride must be 7 mod 64");
 static_assert(kPlane % 4 == 0, "planes are cleared as float4");
@@ -304,10 +305,10 @@ static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 // samples in track-major, channel-minor order; the rows for the next 64
 // are fetched while the current 64 are gridded.  Each sample goes through
 // the CPU gridder's arithmetic (prep_vals); those whose 7x7 footprint
-// touches the tile are compacted, in order, into a per-wave stage as the
-// seven u-weights and the seven products v-weight x visibility, and a
-// 64-bit mask marks the staged samples whose base cell differs from the
-// previous one.  The staged samples are then walked with lanes = the 49
+// touches the tile are kept, in order, a 64-bit mask marks the kept
+// samples whose base cell differs from the previous one, and they pass
+// through a per-wave stage, kHalf at a time, as the seven u-weights and the
+// seven products v-weight x visibility.  Staged samples are walked with lanes = the 49
 // taps of the footprint: each tap lane keeps a running sum while the base
 // cell stays put and hands it to its tile cell when the mask says the
 // cell moves.  A hand-over is a read-modify-write whose write is deferred
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
                                                            long long nitems, Common cm,
                                                            const int *counts, const int *offsets,
                                                            const int *lists,
-                                                           unsigned long long *walk) {
+                                                           unsigned long long *stats) {
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
   __shared__ float4 s_stage[kWaves][kStageN * kStage / 4];
   __shared__ int s_cell[kWaves][64];
@@ -429,78 +430,89 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       const unsigned long long mask = __ballot(keep);
       if (mask == 0) continue;
       const int nv = __popcll(mask);
+      int pos = 0;  // place of this lane's sample among the kept ones
       if (keep) {
-        const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
-        float4 *sp = reinterpret_cast<float4 *>(stg + pos * kStage);
-        const float4 g0 = s_gcf8[2 * sm.itu], g1 = s_gcf8[2 * sm.itu + 1];
-        const float4 h0 = s_gcf8[2 * sm.itv], h1 = s_gcf8[2 * sm.itv + 1];
-        sp[0] = g0;
-        sp[1] = g1;
-        sp[2] = make_float4(h0.x * sm.vr, h0.x * sm.vi, h0.y * sm.vr, h0.y * sm.vi);
-        sp[3] = make_float4(h0.z * sm.vr, h0.z * sm.vi, h0.w * sm.vr, h0.w * sm.vi);
-        sp[4] = make_float4(h1.x * sm.vr, h1.x * sm.vi, h1.y * sm.vr, h1.y * sm.vi);
-        sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, 0.0f, 0.0f);
+        pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
         scell[pos] = (sm.iu & 0xffff) | (sm.iv << 16);
-      }
-      if (lane < kGroup) {
-        float4 *sp = reinterpret_cast<float4 *>(stg + (nv + lane) * kStage);
-#pragma unroll
-        for (int k = 0; k < kStage / 4; ++k) sp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int cellq = scell[lane];  // lane q: base cell of staged sample q
+      const int cellq = scell[lane];  // lane q: base cell of kept sample q
       const int before = scell[lane > 0 ? lane - 1 : 0];
       const int cprev = (lane == 0) ? cur : before;
       const unsigned long long moves = __ballot(lane < nv && cellq != cprev);
       nStaged += nv;
       nMoves += __popcll(moves);
-      // Walk the staged samples kGroup at a time, reading the next group
-      // while the current one accumulates (two register sets in turn); the
-      // zeroed entries after the last sample make a partial group harmless.
-      float wa[kGroup], wb[kGroup];
-      float2 aa[kGroup], ab[kGroup];
-      auto load = [&](float *w, float2 *a, int q0) {
-#pragma unroll
-        for (int j = 0; j < kGroup; ++j) {
-          w[j] = stg[(q0 + j) * kStage + offU];
-          a[j] = *reinterpret_cast<const float2 *>(stg + (q0 + j) * kStage + offV);
+      // The kept samples go through the stage kHalf at a time (a smaller
+      // stage leaves LDS for more waves per CU).  Each part is walked
+      // kGroup samples at a time, reading the next group while the
+      // current one accumulates (two register sets in turn); the zeroed
+      // entries after a part's last sample make a partial group harmless.
+      for (int lo = 0; lo < nv; lo += kHalf) {
+        const int nh = min(kHalf, nv - lo);
+        if (keep && pos >= lo && pos < lo + kHalf) {
+          float4 *sp = reinterpret_cast<float4 *>(stg + (pos - lo) * kStage);
+          const float4 g0 = s_gcf8[2 * sm.itu], g1 = s_gcf8[2 * sm.itu + 1];
+          const float4 h0 = s_gcf8[2 * sm.itv], h1 = s_gcf8[2 * sm.itv + 1];
+          sp[0] = g0;
+          sp[1] = g1;
+          sp[2] = make_float4(h0.x * sm.vr, h0.x * sm.vi, h0.y * sm.vr, h0.y * sm.vi);
+          sp[3] = make_float4(h0.z * sm.vr, h0.z * sm.vi, h0.w * sm.vr, h0.w * sm.vi);
+          sp[4] = make_float4(h1.x * sm.vr, h1.x * sm.vi, h1.y * sm.vr, h1.y * sm.vi);
+          sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, 0.0f, 0.0f);
         }
-      };
-      auto walk = [&](const float *w, const float2 *a, int q0) {
-        const unsigned long long mv = moves >> q0;
+        if (lane < kGroup) {
+          float4 *sp = reinterpret_cast<float4 *>(stg + (nh + lane) * kStage);
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) {
-          if (mv & (1ull << j)) {
-            handover();
-            cur = __builtin_amdgcn_readlane(cellq, q0 + j);
-            ar = 0.0f;
-            ai = 0.0f;
+          for (int k = 0; k < kStage / 4; ++k) sp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float wa[kGroup], wb[kGroup];
+        float2 aa[kGroup], ab[kGroup];
+        auto load = [&](float *w, float2 *a, int q0) {
+#pragma unroll
+          for (int j = 0; j < kGroup; ++j) {
+            w[j] = stg[(q0 + j) * kStage + offU];
+            a[j] = *reinterpret_cast<const float2 *>(stg + (q0 + j) * kStage + offV);
           }
-          ar = fmaf(w[j], a[j].x, ar);
-          ai = fmaf(w[j], a[j].y, ai);
+        };
+        auto walk = [&](const float *w, const float2 *a, int q0) {
+          const unsigned long long mv = moves >> (lo + q0);
+#pragma unroll
+          for (int j = 0; j < kGroup; ++j) {
+            if (mv & (1ull << j)) {
+              handover();
+              cur = __builtin_amdgcn_readlane(cellq, lo + q0 + j);
+              ar = 0.0f;
+              ai = 0.0f;
+            }
+            ar = fmaf(w[j], a[j].x, ar);
+            ai = fmaf(w[j], a[j].y, ai);
+          }
+        };
+        load(wa, aa, 0);
+        for (int q = 0; q < nh; q += 2 * kGroup) {
+          load(wb, ab, q + kGroup);
+          walk(wa, aa, q);
+          if (q + kGroup >= nh) break;
+          load(wa, aa, q + 2 * kGroup);
+          walk(wb, ab, q + kGroup);
         }
-      };
-      load(wa, aa, 0);
-      for (int q = 0; q < nv; q += 2 * kGroup) {
-        load(wb, ab, q + kGroup);
-        walk(wa, aa, q);
-        if (q + kGroup >= nv) break;
-        load(wa, aa, q + 2 * kGroup);
-        walk(wb, ab, q + kGroup);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     handover();
     re[pend] = pvr + par;
     im[pend] = pvi + pai;
-    if (walk && lane == 0) {
-      atomicAdd(&walk[0], (unsigned long long)nStaged);
-      atomicAdd(&walk[1], (unsigned long long)nMoves);
+    if (stats && lane == 0) {
+      atomicAdd(&stats[0], (unsigned long long)nStaged);
+      atomicAdd(&stats[1], (unsigned long long)nMoves);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
