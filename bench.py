@@ -202,7 +202,6 @@ def bench_grid(lib, steps):
             gg.close()
     finally:
         lib.ObitHIPFree(dptr)
-    res["accuracy"] = grid_accuracy(uv, facets, b, e)
     nf = len(facets)
     samples = uv.nrow * nci * nf
     flop = 245.0 * samples  # 49 weight products + 98 complex-MAC FMAs per splat
@@ -227,7 +226,7 @@ def bench_grid(lib, steps):
     })
     log(f"gridding: {ms:.2f} ms/pass, {res['samples_per_s'] / 1e9:.2f} Gsamples/s, "
         f"{res['per_facet_stream_GBps']:.0f} GB/s per-facet-stream model")
-    return res
+    return res, (uv, facets, b, e)
 
 
 def cpu_grid_baseline(seconds):
@@ -351,17 +350,21 @@ def main():
     if args.copy_mib:
         copy_calibration(lib, args.copy_mib)
     if args.grid_only:
-        print(json.dumps({"gridding": bench_grid(lib, args.grid_steps)}), flush=True)
+        grid, _ = bench_grid(lib, args.grid_steps)
+        print(json.dumps({"gridding": grid}), flush=True)
         return
     variant = tuple(int(x) for x in args.variant.split(",")) if args.variant else None
     dft, uv, sky = bench_dft(lib, args.steps, args.warmup, args.sample_rows, variant,
                              accuracy=not args.no_accuracy)
     vt = tuple(dft["variant"][k] for k in ("cpt", "tile", "wg_per_cu"))
     pcie = None if args.no_accuracy else bench_pcie(lib, uv, sky, vt)
-    grid = None if args.no_grid else bench_grid(lib, args.grid_steps)
+    grid, gwork = (None, None) if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)
     if grid is not None and not args.no_cpu:
+        # the CPU leg: Obit's threaded gridder timed, and the oracle's FP64
+        # gridder as the checker of a gridded row sample
         grid["cpu_baseline"] = cpu_grid_baseline(args.cpu_seconds / 2)
+        grid["accuracy"] = grid_accuracy(*gwork)
     if not args.no_accuracy:
         dft_extra = {
             "err_max_vs_fp64_rel_peak": dft["err_max_rel_peak"],
