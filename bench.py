@@ -169,6 +169,40 @@ def grid_accuracy(uv, facets, b, e, rows=2000, nfacet=2):
             "err_rms_vs_fp64_rel_peak": erms}
 
 
+def bench_fft2image(lib, gg, facets):
+    """Flip and FFT-to-image (ObitHIPGridFFT2Image: C2R, quadrant swap,
+    gridding correction, normalisation) of every facet x plane of the
+    gridded workload, timed with HIP events on the gridder's stream."""
+    from obit_amd import _lib
+    f = facets[0]
+    nplane = gg.nplane
+    nbytes = nplane * f.ny * f.nx * 4
+    d = C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(d), nbytes), "malloc image")
+    norm = np.zeros(nplane, np.float32)
+    pn = norm.ctypes.data_as(C.POINTER(C.c_float))
+    try:
+        gg.flip()
+        _lib.check(lib.ObitHIPGridFFT2Image(gg.c, 0, 1, pn, d), "FFT2Image")  # plans, tables
+        st = gg.stream()
+        e0, e1 = _events(lib)
+        lib.ObitCUDAEventRecord(e0, st)
+        for i in range(len(facets)):
+            _lib.check(lib.ObitHIPGridFFT2Image(gg.c, i, 1, pn, d), "FFT2Image")
+        lib.ObitCUDAEventRecord(e1, st)
+        lib.ObitHIPDeviceSynchronize()
+        ms = lib.ObitHIPEventElapsedMs(e0, e1)
+    finally:
+        lib.ObitHIPFree(d)
+    planes = len(facets) * nplane
+    # per plane: half-plane grid read + complex scratch write (trim), the
+    # C2R in place, scratch read + image write (swap, correct)
+    mib = planes * f.ny * (f.nx // 2 + 1) * 8 * 3 / 2**20 + planes * f.ny * f.nx * 4 / 2**20
+    log(f"fft to image: {planes} planes of {f.nx}x{f.ny} in {ms:.1f} ms")
+    return {"planes": planes, "ms": ms, "planes_per_s": planes / (ms * 1e-3),
+            "min_traffic_GBps": mib * 2**20 / (ms * 1e-3) / 1e9}
+
+
 def bench_grid(lib, steps):
     from obit_amd import synth, _lib
     from obit_amd.gridder import GPUGrid, plane_of_channel
@@ -198,6 +232,7 @@ def bench_grid(lib, steps):
             res["lists"] = dict(zip(("lists", "non_empty", "entries", "items", "longest", "split",
                                      "tile_samples", "cell_moves"), list(st8)))
             log(f"gridding lists: {res['lists']}")
+            res["fft2image"] = bench_fft2image(lib, gg, facets)
         finally:
             gg.close()
     finally:

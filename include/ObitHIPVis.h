@@ -98,6 +98,17 @@ int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out);
  * (first of the 7 rows), itu, itv (GCF block 0..100) and vis[2*(..)]. */
 int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis,
                          int *iu, int *iv, int *itu, int *itv, float *vis);
+/* ObitUVGridFFT2Im (src/ObitUVGrid.c:673-834) on the device for every plane
+ * of facet ifacet, after ObitCUDAGridFlip: trim and centre the planes, one
+ * batched C2R FFT (unnormalised, as FFTW's backward transform), the
+ * quadrant swap of ObitFArray2DCenter, the gridding correction of
+ * GridCorrFn applied as ObitFArrayMulColRow does, and the normalisation.
+ * isBeam=1: beamNorm[p] receives each plane's centre value x corrections
+ * and normalises it; isBeam=0: beamNorm[p] (from the beam) normalises the
+ * image.  d_image: device [nplane][ny][nx] floats.  Facets must be square
+ * and even-sized (the reference's correction indexes rows and columns
+ * alike).  Returns after the images are written. */
+int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamNorm, float *d_image);
 /* Statistics of the last gridding pass on iGPU (stats has 8 entries):
  * [0] tile lists, [1] non-empty lists, [2] list entries (track x tile),
  * [3] work items, [4] longest list, [5] lists split into several work

@@ -137,6 +137,7 @@ _PROTOS = {
     "ObitHIPGrid2HostPlanes": (_I, [_P, _L, _P]),
     "ObitHIPGridPrepDebug": (_I, [_P, _L, _P, _LL, _P, _P, _P, _P, _P]),
     "ObitHIPGridListStats": (_I, [_P, _L, _P]),
+    "ObitHIPGridFFT2Image": (_I, [_P, _L, _I, _FP, _P]),
     "ObitHIPGridStream": (_P, [_P, _L]),
     # host classes (csrc/host.h)
     "ObitHostDFTCreate": (_P, []),
@@ -152,6 +153,7 @@ _PROTOS = {
     "ObitHostGrid2Host": (_I, [_P, _I, _P]),
     "ObitHostGridShutdown": (None, [_P]),
     "ObitHostPlanesFromBins": (None, [_I, _I, _IP, _IP, _I, _IP]),
+    "ObitHostGridCorrFn": (None, [_I, _I, _FP, _I, _I, _FP]),
 }
 
 _lib = None

@@ -21,6 +21,9 @@
 // split over several items.
 #include "common.h"
 #include "ObitCUDAGrid.h"
+#include "host.h"
+
+#include <hipfft/hipfft.h>
 #include "ObitCUDAUtil.h"
 #include "ObitHIPVis.h"
 
@@ -585,6 +588,27 @@ __global__ void k_trim_centre(const FacetDev *facets, int fi, int nplane, float 
   }
 }
 
+// ObitFArray2DCenter's quadrant swap of the C2R output (in-place layout,
+// rows of 2*(nx/2+1) floats), then ObitFArrayMulColRow's correction as the
+// reference applies it to its square images: pixel (x, y) times
+// xCorr[y]/norm and yCorr[x] (src/ObitFArray.c:3116-3123).
+__global__ void k_center_correct(const float *fft, int nx, int ny, int nplane, const float *xc,
+                                 const float *yc, const float *invNorm, float *img) {
+  const long long total = (long long)nplane * ny * nx;
+  const int pitch = 2 * (nx / 2 + 1);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx);
+    const long long yp = i / nx;
+    const int y = (int)(yp % ny);
+    const int p = (int)(yp / ny);
+    const int xs = (x + nx / 2) % nx, ys = (y + ny / 2) % ny;
+    const float v = fft[((size_t)p * ny + ys) * pitch + xs];
+    const float row = xc[y] * invNorm[p];
+    img[i] = v * row * yc[x];
+  }
+}
+
 // Per-sample outputs of prep_sample for the parity tests.
 __global__ void k_prep_debug(const float *vis, long long nrow, const FacetDev *facets, int fi,
                              Common cm, int *iu, int *iv, int *itu, int *itv, float *vout) {
@@ -620,6 +644,13 @@ struct GpuState {
   int2 *d_items = nullptr;
   long long itemCap = 0;
   unsigned long long *d_walk = nullptr;  // staged samples, base-cell moves of the last pass
+  // FFT to images: C2R plan for (nx, ny, nplane), complex scratch, corrections
+  hipfftHandle fftPlan = 0;
+  int fftNx = 0, fftNy = 0, fftNplane = 0;
+  float *d_fft = nullptr;
+  size_t fftCap = 0;
+  float *d_corr = nullptr;  // xCorr[n], yCorr[n], 1/norm[nplane]
+  size_t corrCap = 0;
   std::vector<int> h_counts, h_offsets;
   std::vector<int2> h_items;
   // tables
@@ -1052,6 +1083,8 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
     (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
     (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis); (void)hipFree(gs->d_walk);
+    if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
+    (void)hipFree(gs->d_fft); (void)hipFree(gs->d_corr);
     if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
     (void)hipStreamDestroy(gs->stream);
     obit::grid::GridCtx *c = obit::grid::get_ctx(cudaInfo, false);
@@ -1131,6 +1164,94 @@ int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out) {
   }
   (void)hipFree(d);
   return rc;
+}
+
+int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamNorm, float *d_image) {
+  if (!in || !in->h_gridInfo || !beamNorm || !d_image) return obit::fail(obit::kInvalid, "null argument");
+  const long iGPU = in->FacetGPU ? in->FacetGPU[ifacet] : 0;
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  if (!gs || !gs->ready) return obit::fail(obit::kState, "gridder not initialised");
+  OBIT_HIP_TRY(hipSetDevice(gs->device));
+  int k = -1;
+  for (size_t i = 0; i < gs->facets.size(); ++i) if (gs->facets[i] == ifacet) k = (int)i;
+  if (k < 0) return obit::fail(obit::kInvalid, "facet %ld not on GPU index %ld", ifacet, iGPU);
+  const auto &f = gs->h_facets[k];
+  const int nx = f.nx, ny = f.ny, nplane = in->h_gridInfo->nplane;
+  const GridInfo *gi = in->h_gridInfo;
+  if (nx != ny || (nx & 1))
+    return obit::fail(obit::kInvalid, "FFT2Image needs a square, even facet (got %d x %d): the "
+                      "reference's correction indexes rows and columns alike", nx, ny);
+  if (!gi->h_convfn) return obit::fail(obit::kState, "no convolving function");
+  // trim and centre the (flipped) planes into the complex scratch
+  const size_t nxo = (size_t)nx / 2 + 1;
+  const size_t need = (size_t)nplane * ny * nxo * 2;
+  if (need > gs->fftCap) {
+    (void)hipFree(gs->d_fft);
+    gs->d_fft = nullptr;
+    OBIT_HIP_TRY(hipMalloc(&gs->d_fft, sizeof(float) * need));
+    gs->fftCap = need;
+  }
+  {
+    const long long total = (long long)nplane * ny * nxo;
+    const int nb = (int)std::min<long long>((total + 255) / 256, (long long)gs->nCU * 16);
+    hipLaunchKernelGGL(obit::grid::k_trim_centre, dim3(std::max(nb, 1)), dim3(256), 0, gs->stream,
+                       gs->d_facets, k, nplane, gs->d_fft);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  // one batched in-place C2R (FFTW's unnormalised backward transform)
+  if (!gs->fftPlan || gs->fftNx != nx || gs->fftNy != ny || gs->fftNplane != nplane) {
+    if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
+    gs->fftPlan = 0;
+    int n[2] = {ny, nx}, inembed[2] = {ny, (int)nxo}, onembed[2] = {ny, 2 * (int)nxo};
+    hipfftResult r = hipfftPlanMany(&gs->fftPlan, 2, n, inembed, 1, ny * (int)nxo, onembed, 1,
+                                    ny * 2 * (int)nxo, HIPFFT_C2R, nplane);
+    if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftPlanMany: %d", (int)r);
+    gs->fftNx = nx;
+    gs->fftNy = ny;
+    gs->fftNplane = nplane;
+  }
+  if (hipfftSetStream(gs->fftPlan, gs->stream) != HIPFFT_SUCCESS)
+    return obit::fail(obit::kHipError, "hipfftSetStream");
+  {
+    hipfftResult r = hipfftExecC2R(gs->fftPlan, (hipfftComplex *)gs->d_fft, (hipfftReal *)gs->d_fft);
+    if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftExecC2R: %d", (int)r);
+  }
+  // gridding corrections (GridCorrFn) and normalisation
+  std::vector<float> corr((size_t)nx + ny + nplane);
+  ObitHostGridCorrFn(nx, nx / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell, corr.data());
+  ObitHostGridCorrFn(ny, ny / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell, corr.data() + nx);
+  if (isBeam) {
+    // centre pixel of each plane after the swap = element (0, 0) before it
+    std::vector<float> c0(nplane);
+    OBIT_HIP_TRY(hipMemcpy2DAsync(c0.data(), sizeof(float), gs->d_fft, sizeof(float) * ny * 2 * nxo,
+                                  sizeof(float), nplane, hipMemcpyDeviceToHost, gs->stream));
+    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+    for (int p = 0; p < nplane; ++p) {
+      beamNorm[p] = c0[p] * corr[nx / 2] * corr[nx + ny / 2];
+      if (beamNorm[p] == 0.0f) return obit::fail(obit::kInvalid, "beam peak is zero in plane %d", p);
+    }
+  }
+  for (int p = 0; p < nplane; ++p)
+    corr[(size_t)nx + ny + p] = (float)(1.0 / std::max(1.0e-20, (double)beamNorm[p]));
+  if (corr.size() > gs->corrCap) {
+    (void)hipFree(gs->d_corr);
+    gs->d_corr = nullptr;
+    OBIT_HIP_TRY(hipMalloc(&gs->d_corr, sizeof(float) * corr.size()));
+    gs->corrCap = corr.size();
+  }
+  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_corr, corr.data(), sizeof(float) * corr.size(),
+                              hipMemcpyHostToDevice, gs->stream));
+  {
+    const long long total = (long long)nplane * ny * nx;
+    const int nb = (int)std::min<long long>((total + 255) / 256, (long long)gs->nCU * 16);
+    hipLaunchKernelGGL(obit::grid::k_center_correct, dim3(std::max(nb, 1)), dim3(256), 0, gs->stream,
+                       gs->d_fft, nx, ny, nplane, gs->d_corr, gs->d_corr + nx,
+                       gs->d_corr + nx + ny, d_image);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  // corr is read by the async copy above
+  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+  return obit::kOK;
 }
 
 int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis, int *iu,

@@ -94,6 +94,12 @@ void ObitHostGridShutdown(CUDAGridInfo *c);
 void ObitHostPlanesFromBins(int nchanIF, int nbin, const int *bChan, const int *eChan,
                             int cpuParity, int *planeOfChan);
 
+/* GridCorrFn (src/ObitUVGrid.c:1612-1662): out[n] = the gridding
+ * correction of an n-pixel axis centred on 1-relative pixel icent, for the
+ * convolving-function table convfn (convWidth*convNperCell+1 entries). */
+void ObitHostGridCorrFn(int n, int icent, const float *convfn, int convWidth, int convNperCell,
+                        float *out);
+
 #ifdef __cplusplus
 }
 #endif

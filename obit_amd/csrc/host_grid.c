@@ -61,6 +61,53 @@ void ObitHostConvFuncSph(float *convfn)
   for (i = 1; i <= bias - 1; i++) convfn[bias - i] = convfn[bias + i];
 }
 
+/* GridCorrFn (src/ObitUVGrid.c:1612-1662): the gridding correction for one
+ * image axis of n pixels centred on 1-relative pixel icent, the inverse of
+ * the cosine transform of the tabulated convolving function, normalised
+ * to 1 at pixel n/2.  The transform is evaluated as the reference does: an
+ * FP32 complex phase ramp applied to every table entry once per pixel. */
+void ObitHostGridCorrFn(int n, int icent, const float *convfn, int convWidth, int convNperCell,
+                        float *out)
+{
+  const int size = convWidth * convNperCell + 1;
+  const int bias = convWidth * (convNperCell / 2) + 1;
+  /* the reference forms these in double (double literals) and stores floats */
+  const float p4 = (float)(2.0 * 3.14159265358979323846 / (convNperCell * (double)n));
+  const float p1 = (float)((double)p4 * (1.0 - (double)icent) * (1.0 - (double)bias));
+  const float p2 = (float)((double)p4 * (1.0 - (double)icent));
+  const float p3 = (float)((double)p4 * (1.0 - (double)bias));
+  float *data = (float *)malloc(sizeof(float) * 4 * (size_t)size);
+  float *ramp = data + 2 * size;
+  int i, j;
+  if (!data) return;
+  for (i = 0; i < size; i++) {
+    float phase = p1 + (float)i * p2;
+    data[2 * i] = (float)((double)convfn[i] * cos((double)phase));
+    data[2 * i + 1] = (float)((double)convfn[i] * sin((double)phase));
+    phase = p3 + (float)i * p4;
+    ramp[2 * i] = (float)cos(phase);
+    ramp[2 * i + 1] = (float)sin(phase);
+  }
+  for (i = 0; i < n; i++) {
+    float sumre = 0.0f;
+    for (j = 0; j < 2 * size; j += 2) {
+      float tr, ti;
+      sumre += data[j];
+      tr = data[j] * ramp[j] - data[j + 1] * ramp[j + 1];
+      ti = data[j] * ramp[j + 1] + data[j + 1] * ramp[j];
+      data[j] = tr;
+      data[j + 1] = ti;
+    }
+    out[i] = sumre;
+  }
+  {
+    const float c = (float)(1.0 / (double)out[n / 2]);  /* ObitFArraySMul */
+    for (i = 0; i < n; i++) out[i] *= c;
+    for (i = 0; i < n; i++) out[i] = 1.0f / out[i];      /* ObitFArraySDiv */
+  }
+  free(data);
+}
+
 void ObitHostPlanesFromBins(int nchanIF, int nbin, const int *bChan, const int *eChan,
                             int cpuParity, int *planeOfChan)
 {

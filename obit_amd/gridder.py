@@ -101,6 +101,26 @@ class GPUGrid:
         _lib.check(self.lib.ObitHIPGrid2HostPlanes(self.c, ifacet, _lib.ptr(out)), "Grid2HostPlanes")
         return out[..., 0] + 1j * out[..., 1]
 
+    def fft_image(self, ifacet: int, is_beam: bool, beam_norm=None):
+        """Planes of a flipped facet -> corrected, normalised images on the
+        device (ObitHIPGridFFT2Image), copied back: (float32 [nplane, ny, nx],
+        per-plane beam normalisation)."""
+        f = self.facets[ifacet]
+        norm = np.zeros(self.nplane, np.float32)
+        if not is_beam:
+            norm[:] = np.asarray(beam_norm, np.float32)
+        out = np.empty((self.nplane, f.ny, f.nx), np.float32)
+        d = C.c_void_p()
+        _lib.check(self.lib.ObitHIPMalloc(C.byref(d), out.nbytes), "malloc image")
+        try:
+            _lib.check(self.lib.ObitHIPGridFFT2Image(self.c, ifacet, int(is_beam),
+                                                     norm.ctypes.data_as(C.POINTER(C.c_float)), d),
+                       "FFT2Image")
+            _lib.check(self.lib.ObitHIPMemcpyD2H(_lib.ptr(out), d, out.nbytes), "image to host")
+        finally:
+            self.lib.ObitHIPFree(d)
+        return out, norm
+
     def prep_debug(self, ifacet: int, buffer: np.ndarray):
         """Per-sample (iu, iv, itu, itv, vis) as the device computes them."""
         buffer = np.ascontiguousarray(buffer, dtype=np.float32)

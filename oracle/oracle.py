@@ -283,3 +283,47 @@ def trim_centre(grid, facet, dtype=np.complex128):
     g = grid[:lrow * ny].reshape(ny, lrow // 2, 2)
     g = g[:, 3:3 + nx // 2 + 1, 0] + 1j * g[:, 3:3 + nx // 2 + 1, 1]
     return np.roll(g, ny // 2, axis=0).astype(dtype)
+
+
+# ------------------------------------------------------------ FFT to image
+
+def grid_corr_fn(n, icent, convfn, width=7, nper=100):
+    """GridCorrFn (src/ObitUVGrid.c:1612-1662) by its definition, in FP64:
+    the cosine transform of the tabulated convolving function,
+    sum_i convfn[i] cos(2 pi (i + 1 - bias)(x + 1 - icent) / (nper n)),
+    normalised to 1 at pixel n/2 and inverted.  (The reference evaluates
+    the same sum with an FP32 phase-ramp recurrence.)"""
+    size = width * nper + 1
+    bias = width * (nper // 2) + 1
+    i = np.arange(size, dtype=np.float64)
+    x = np.arange(n, dtype=np.float64)
+    ph = 2.0 * np.pi * np.outer(x + 1.0 - icent, i + 1.0 - bias) / (nper * n)
+    s = (np.asarray(convfn[:size], np.float64)[None, :] * np.cos(ph)).sum(axis=1)
+    s = s / s[n // 2]
+    return 1.0 / s
+
+
+def fft2image(planes, convfn, is_beam, beam_norm=None, corr=None):
+    """ObitUVGridFFT2Im (src/ObitUVGrid.c:673-834) in FP64 with numpy's FFT
+    standing in for FFTW: planes [nplane, ny, nx/2+1] complex (trimmed and
+    centred as ThreadMerge leaves them) -> unnormalised C2R, quadrant swap
+    (ObitFArray2DCenter), correction applied as ObitFArrayMulColRow does
+    (pixel (x, y) times xCorr[y] / norm times yCorr[x]) and the beam's
+    centre-value normalisation.  corr=(xCorr, yCorr) replaces the FP64
+    correction functions.  Returns (images [nplane, ny, nx], norms)."""
+    planes = np.asarray(planes)
+    nplane, ny, nxo = planes.shape
+    nx = 2 * (nxo - 1)
+    if corr is None:
+        xc = grid_corr_fn(nx, nx // 2 + 1, convfn)
+        yc = grid_corr_fn(ny, ny // 2 + 1, convfn)
+    else:
+        xc, yc = (np.asarray(c, np.float64) for c in corr)
+    img = np.fft.irfft2(planes, s=(ny, nx), axes=(1, 2)) * (nx * ny)
+    img = np.roll(img, (ny // 2, nx // 2), axis=(1, 2))
+    if is_beam:
+        norms = img[:, ny // 2, nx // 2] * xc[nx // 2] * yc[ny // 2]
+    else:
+        norms = np.asarray(beam_norm, np.float64)
+    out = img * (xc[None, :, None] / norms[:, None, None]) * yc[None, None, :]
+    return out, norms

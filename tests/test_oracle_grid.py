@@ -171,3 +171,68 @@ def test_flip_closed_forms(oracle):
     a = c0[1:]                                 # v = -ny/2+1 .. ny/2-1
     assert np.abs(c0).max() > 0
     assert np.allclose(a, np.conj(a[::-1]), rtol=0, atol=1e-12 * np.abs(c0).max())
+
+
+def test_grid_corr_fn_symmetry_and_recurrence(oracle):
+    """The correction is 1 at the centre and symmetric about it; the
+    library's statement (the reference's FP32 phase-ramp recurrence,
+    ObitHostGridCorrFn) follows the FP64 definition."""
+    import ctypes as C
+    from obit_amd import _lib
+    cf, _ = oracle.convfunc()
+    for n in (256, 2048):
+        c = oracle.grid_corr_fn(n, n // 2 + 1, cf)
+        assert c[n // 2] == 1.0
+        assert np.allclose(c[n // 2 + 1:], c[n // 2 - 1:0:-1], rtol=1e-12)
+        assert (np.diff(c[n // 2:]) > 0).all()  # the taper grows towards the edge
+        out = np.zeros(n, np.float32)
+        _lib.load().ObitHostGridCorrFn(n, n // 2 + 1, cf.ctypes.data_as(C.POINTER(C.c_float)), 7, 100,
+                                       out.ctypes.data_as(C.POINTER(C.c_float)))
+        rel = np.abs(out / c - 1.0)
+        print(f"n={n}: max rel diff of the FP32 recurrence {rel.max():.2e}, edge value {c[0]:.4f}")
+        # the recurrence rotates each phasor n times in FP32: its error grows
+        # with the pixel index, as the reference's own does
+        assert rel.max() < 5e-7 * n
+
+
+def test_fft2image_recovers_an_offset_point_source(oracle):
+    """A unit point source away from the phase centre, gridded (FP64) and
+    turned into an image by the FFT-to-image restatement, comes back at the
+    right pixel with unit flux relative to the beam: the correction undoes
+    the gridding kernel's taper (without it the peak is visibly low)."""
+    nx = 128
+    f = _facet(nx=nx, cell_arcsec=10.0)
+    cell = float(np.float32(np.pi / 180 * float(np.float32(f.cell_deg))))
+    rng = np.random.default_rng(5)
+    n = 4000
+    umax = 0.35 / cell
+    u = rng.uniform(-umax, umax, n)
+    v = rng.uniform(-umax, umax, n)
+    kx, ky = 23, -17                     # pixel offsets of the source
+    l0, m0 = kx * cell, ky * cell
+    hdr = np.stack([u, v, np.zeros(n), np.full(n, 258.0), np.zeros(n)], axis=1)
+    fa = synth.freq_table(1.4e9, 1, 1, 1e6)
+    uvb = synth.make_uv(hdr, 1, 1, 1, 1.4e9, fa, rng, crval_stokes=1.0)
+    uvb.buffer[:, 5] = 1.0
+    uvb.buffer[:, 6] = 0.0
+    uvb.buffer[:, 7] = 1.0
+    uvs = synth.make_uv(hdr, 1, 1, 1, 1.4e9, fa, rng, crval_stokes=1.0)
+    uf = uvs.buffer[:, 0].astype(np.float64)
+    vf = uvs.buffer[:, 1].astype(np.float64)
+    ph = 2 * np.pi * (uf * l0 + vf * m0)
+    uvs.buffer[:, 5] = np.cos(ph)
+    uvs.buffer[:, 6] = np.sin(ph)
+    uvs.buffer[:, 7] = 1.0
+    cf, _ = oracle.convfunc()
+    gb = oracle.trim_centre(oracle.grid_fp64(uvb, f), f)[None]
+    gs = oracle.trim_centre(oracle.grid_fp64(uvs, f), f)[None]
+    beam, norm = oracle.fft2image(gb, cf, True)
+    img, _ = oracle.fft2image(gs, cf, False, norm)
+    assert abs(beam[0, nx // 2, nx // 2] - 1.0) < 1e-12
+    py, px = np.unravel_index(np.argmax(img[0]), img[0].shape)
+    assert (abs(px - nx // 2), abs(py - nx // 2)) == (abs(kx), abs(ky))
+    peak = img[0, py, px]
+    print(f"offset source recovered at ({px - nx // 2}, {py - nx // 2}) with flux {peak:.5f}")
+    assert abs(peak - 1.0) < 2e-3
+    xc = oracle.grid_corr_fn(nx, nx // 2 + 1, cf)
+    assert peak / (xc[py] * xc[px]) < 0.99  # the uncorrected image is tapered there
