@@ -649,8 +649,10 @@ struct GpuState {
   int fftNx = 0, fftNy = 0, fftNplane = 0;
   float *d_fft = nullptr;
   size_t fftCap = 0;
-  float *d_corr = nullptr;  // xCorr[n], yCorr[n], 1/norm[nplane]
+  float *d_corr = nullptr;  // xCorr[nx], yCorr[ny], 1/norm[nplane]
   size_t corrCap = 0;
+  std::vector<float> h_corr;  // xCorr, yCorr of the last (nx, ny) (GridCorrFn)
+  int corrNx = 0, corrNy = 0;
   std::vector<int> h_counts, h_offsets;
   std::vector<int2> h_items;
   // tables
@@ -1216,10 +1218,18 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
     hipfftResult r = hipfftExecC2R(gs->fftPlan, (hipfftComplex *)gs->d_fft, (hipfftReal *)gs->d_fft);
     if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftExecC2R: %d", (int)r);
   }
-  // gridding corrections (GridCorrFn) and normalisation
-  std::vector<float> corr((size_t)nx + ny + nplane);
-  ObitHostGridCorrFn(nx, nx / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell, corr.data());
-  ObitHostGridCorrFn(ny, ny / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell, corr.data() + nx);
+  // gridding corrections (GridCorrFn, computed once per size) and normalisation
+  if (gs->corrNx != nx || gs->corrNy != ny) {
+    gs->h_corr.assign((size_t)nx + ny, 0.0f);
+    ObitHostGridCorrFn(nx, nx / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell, gs->h_corr.data());
+    ObitHostGridCorrFn(ny, ny / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell,
+                       gs->h_corr.data() + nx);
+    gs->corrNx = nx;
+    gs->corrNy = ny;
+    gs->corrCap = 0;  // upload again below
+  }
+  std::vector<float> corr(gs->h_corr);
+  corr.resize((size_t)nx + ny + nplane);
   if (isBeam) {
     // centre pixel of each plane after the swap = element (0, 0) before it
     std::vector<float> c0(nplane);
@@ -1238,8 +1248,10 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
     gs->d_corr = nullptr;
     OBIT_HIP_TRY(hipMalloc(&gs->d_corr, sizeof(float) * corr.size()));
     gs->corrCap = corr.size();
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->d_corr, corr.data(), sizeof(float) * (nx + ny),
+                                hipMemcpyHostToDevice, gs->stream));
   }
-  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_corr, corr.data(), sizeof(float) * corr.size(),
+  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_corr + nx + ny, corr.data() + nx + ny, sizeof(float) * nplane,
                               hipMemcpyHostToDevice, gs->stream));
   {
     const long long total = (long long)nplane * ny * nx;
