@@ -63,6 +63,14 @@ class HostFacet(C.Structure):
     ]
 
 
+class HostFieldGeom(C.Structure):
+    _fields_ = [
+        ("imCrval", C.c_double * 2), ("imCrota", C.c_float), ("do3D", C.c_int),
+        ("xPxOff", C.c_float), ("yPxOff", C.c_float),
+        ("uvCrval", C.c_double * 2), ("uvCrota", C.c_float),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _L = C.c_long
@@ -154,6 +162,7 @@ _PROTOS = {
     "ObitHostGridShutdown": (None, [_P]),
     "ObitHostPlanesFromBins": (None, [_I, _I, _IP, _IP, _I, _IP]),
     "ObitHostGridCorrFn": (None, [_I, _I, _FP, _I, _I, _FP]),
+    "ObitHostLoadCompsPoint": (_I, [C.POINTER(HostFieldGeom), _FP, _I, _I, _F, _FP]),
 }
 
 _lib = None

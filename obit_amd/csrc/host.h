@@ -94,6 +94,26 @@ void ObitHostGridShutdown(CUDAGridInfo *c);
 void ObitHostPlanesFromBins(int nchanIF, int nbin, const int *bChan, const int *eChan,
                             int cpuParity, int *planeOfChan);
 
+/* The geometry of one image field against the uv data that
+ * ObitSkyModelLoadComps reads (ObitImageDesc / ObitUVDesc fields). */
+typedef struct {
+  double imCrval[2];     /* image reference RA, Dec (deg) */
+  float imCrota;         /* image rotation (crota of the Dec axis, deg) */
+  int do3D;              /* 1: 3-D image (the only kind restated) */
+  float xPxOff, yPxOff;  /* 2-D reference-pixel offsets (0 for 3-D) */
+  double uvCrval[2];     /* uv phase centre RA, Dec (deg) */
+  float uvCrota;         /* uv rotation (deg) */
+} ObitHostFieldGeom;
+
+/* ObitSkyModelLoadComps for the point components of one 3-D image field
+ * (src/ObitSkyModel.c:1889-1990 with ObitUVDescShiftPhase,
+ * ObitSkyGeomShiftSIN and ObitUVDescShift3DMatrix, in the reference's
+ * float/double arithmetic): cc rows [flux, deltaX, deltaY (deg), ...] of
+ * ccStride floats -> comps rows [flux*factor, x, y, z] (Obit's 2 pi l,
+ * 2 pi m, 2 pi (n-1)).  Returns ncc, or -1 for a 2-D field or bad input. */
+int ObitHostLoadCompsPoint(const ObitHostFieldGeom *g, const float *cc, int ncc, int ccStride,
+                           float factor, float *comps);
+
 /* GridCorrFn (src/ObitUVGrid.c:1612-1662): out[n] = the gridding
  * correction of an n-pixel axis centred on 1-relative pixel icent, for the
  * convolving-function table convfn (convWidth*convNperCell+1 entries). */

@@ -16,6 +16,7 @@
 #include "ObitCUDAUtil.h"
 #include "ObitHIPVis.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -243,3 +244,129 @@ void ObitHostDFTShutdown(GPUInfo *g)
   free(g->d_data);
   free(g);
 }
+
+/* ------------------------------------------------------------------ */
+/* Component table of one image field (ObitSkyModelLoadComps)          */
+/* ------------------------------------------------------------------ */
+
+/* glib's G_PI and Obit's DG2RAD macro (include/ObitTypes.h:43), kept as
+ * macros: DG2RAD expands textually, so "a * DG2RAD" is (a * pi) / 180. */
+#define OBIT_G_PI 3.1415926535897932384626433832795028841971693993751
+#define OBIT_DG2RAD OBIT_G_PI / 180.0
+
+/* ObitSkyGeomShiftSIN (src/ObitSkyGeom.c:177-205). */
+static void geom_shift_sin(double ra, double dec, float rotate, double xra, double xdec,
+                           float dxyzc[3])
+{
+  float maprr, xshift, yshift;
+  double xxshft, yyshft, dzctmp;
+  maprr = rotate * OBIT_DG2RAD;
+  xxshft = cos(OBIT_DG2RAD * xdec) * sin(OBIT_DG2RAD * (xra - ra));
+  yyshft = sin(OBIT_DG2RAD * xdec) * cos(OBIT_DG2RAD * dec) -
+           cos(OBIT_DG2RAD * xdec) * sin(OBIT_DG2RAD * dec) * cos(OBIT_DG2RAD * (xra - ra));
+  xshift = (cos(maprr) * xxshft + sin(maprr) * yyshft);
+  yshift = (-sin(maprr) * xxshft + cos(maprr) * yyshft);
+  dxyzc[0] = xshift;
+  dxyzc[1] = yshift;
+  dzctmp = 1.0 - (dxyzc[0] * dxyzc[0]) - (dxyzc[1] * dxyzc[1]);
+  dzctmp = sqrt(dzctmp);
+  dxyzc[0] = 2.0 * OBIT_G_PI * dxyzc[0];
+  dxyzc[1] = 2.0 * OBIT_G_PI * dxyzc[1];
+  dxyzc[2] = 2.0 * OBIT_G_PI * (dzctmp - 1.0);
+}
+
+/* The uv re-projection matrix of ObitUVDescShift3DMatrix for a 3-D image
+ * (src/ObitUVDesc.c:882-975, imDesc->do3D branch); returns do3Dmul. */
+static int geom_shift3d_umat(double ra, double dec, float urotat, double xra, double xdec,
+                             float mrotat, float xPxOff, float yPxOff, float umat[3][3])
+{
+  double rm[3][3], x[3][3], t[3][3], r, sa, ca, sd, cd, sd0, cd0;
+  int i, j, k, do3Dmul;
+  for (i = 0; i < 3; i++)
+    for (j = 0; j < 3; j++) rm[i][j] = 0.0;
+  rm[2][2] = 1.0;
+  do3Dmul = (mrotat != 0.0) || (urotat != 0.0) || (fabs(xra - ra) > 1.0e-10) ||
+            (fabs(xdec - dec) > 1.0e-10) || (fabs(xPxOff) > 1.0e-20) || (fabs(yPxOff) > 1.0e-20);
+  sa = sin(OBIT_DG2RAD * (xra - ra));
+  ca = cos(OBIT_DG2RAD * (xra - ra));
+  sd = sin(OBIT_DG2RAD * xdec);
+  cd = cos(OBIT_DG2RAD * xdec);
+  sd0 = sin(OBIT_DG2RAD * dec);
+  cd0 = cos(OBIT_DG2RAD * dec);
+  /* map + */
+  r = OBIT_DG2RAD * mrotat;
+  rm[0][0] = cos(r);
+  rm[1][1] = rm[0][0];
+  rm[1][0] = -sin(r);
+  rm[0][1] = sin(r);
+  /* forward matrix */
+  x[0][0] = ca;
+  x[1][0] = -sd * sa;
+  x[2][0] = cd * sa;
+  x[0][1] = sd0 * sa;
+  x[1][1] = cd * cd0 + sd * sd0 * ca;
+  x[2][1] = sd * cd0 - cd * sd0 * ca;
+  x[0][2] = -cd0 * sa;
+  x[1][2] = cd * sd0 - sd * cd0 * ca;
+  x[2][2] = sd * sd0 + cd * cd0 * ca;
+  for (i = 0; i < 3; i++)
+    for (j = 0; j < 3; j++) {
+      t[j][i] = 0.0;
+      for (k = 0; k < 3; k++) t[j][i] += x[k][i] * rm[j][k];
+    }
+  /* uv - */
+  r = OBIT_DG2RAD * urotat;
+  rm[0][0] = cos(r);
+  rm[1][1] = rm[0][0];
+  rm[1][0] = sin(r);
+  rm[0][1] = -sin(r);
+  for (i = 0; i < 3; i++)
+    for (j = 0; j < 3; j++) {
+      r = 0.0;
+      for (k = 0; k < 3; k++) r += rm[k][i] * t[j][k];
+      umat[j][i] = r;
+    }
+  return do3Dmul;
+}
+
+int ObitHostLoadCompsPoint(const ObitHostFieldGeom *g, const float *cc, int ncc, int ccStride,
+                           float factor, float *comps)
+{
+  float dxyzc[3], umat[3][3], xp[3], xyz[3], konst, ssrot, ccrot, xxoff, yyoff, zzoff;
+  int j, do3Dmul;
+  if (!g || !cc || !comps || ncc < 0 || ccStride < 3) return -1;
+  if (!g->do3D) return -1;  /* 2-D fields (tangent-point offsets) are not restated */
+  /* ObitUVDescShiftPhase for a 3-D image: the shift to the image's
+   * reference position (src/ObitUVDesc.c:735-781) */
+  geom_shift_sin(g->uvCrval[0], g->uvCrval[1], g->imCrota, g->imCrval[0], g->imCrval[1], dxyzc);
+  ssrot = sin(OBIT_DG2RAD * (g->uvCrota - g->imCrota));
+  ccrot = cos(OBIT_DG2RAD * (g->uvCrota - g->imCrota));
+  xxoff = dxyzc[0] * ccrot + dxyzc[1] * ssrot;
+  yyoff = dxyzc[1] * ccrot - dxyzc[0] * ssrot;
+  zzoff = dxyzc[2];
+  do3Dmul = geom_shift3d_umat(g->uvCrval[0], g->uvCrval[1], g->uvCrota, g->imCrval[0], g->imCrval[1],
+                              g->imCrota, g->xPxOff, g->yPxOff, umat);
+  konst = OBIT_DG2RAD * 2.0 * OBIT_G_PI;
+  for (j = 0; j < ncc; j++) {
+    const float *a = cc + (size_t)j * ccStride;
+    float *row = comps + (size_t)j * 4;
+    row[0] = a[0] * factor;
+    xp[0] = (a[1] + 0.0f) * konst;  /* 3-D: no reference-pixel offsets */
+    xp[1] = (a[2] + 0.0f) * konst;
+    xp[2] = 0.0f;
+    if (do3Dmul) {
+      xyz[0] = xp[0] * umat[0][0] + xp[1] * umat[1][0];
+      xyz[1] = xp[0] * umat[0][1] + xp[1] * umat[1][1];
+      xyz[2] = xp[0] * umat[0][2] + xp[1] * umat[1][2];
+    } else {
+      xyz[0] = ccrot * xp[0] + ssrot * xp[1];
+      xyz[1] = ccrot * xp[1] - ssrot * xp[0];
+      xyz[2] = xp[2];
+    }
+    row[1] = xyz[0] + xxoff;
+    row[2] = xyz[1] + yyoff;
+    row[3] = xyz[2] + zzoff;
+  }
+  return ncc;
+}
+

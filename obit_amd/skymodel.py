@@ -38,6 +38,26 @@ def host_uvdesc(uv: UVData):
     return d, (fa, fs)
 
 
+def load_comps_point(cc, im_crval, im_crota, uv_crval, uv_crota=0.0, factor=1.0) -> np.ndarray:
+    """Component table of one 3-D image field from its CC rows
+    [flux, deltaX, deltaY (deg)], as ObitSkyModelLoadComps builds it
+    (ObitHostLoadCompsPoint): float32 [n, 4] rows [flux*factor, x, y, z]."""
+    cc = np.ascontiguousarray(cc, dtype=np.float32)
+    g = _lib.HostFieldGeom()
+    g.imCrval[0], g.imCrval[1] = float(im_crval[0]), float(im_crval[1])
+    g.imCrota = float(im_crota)
+    g.do3D = 1
+    g.uvCrval[0], g.uvCrval[1] = float(uv_crval[0]), float(uv_crval[1])
+    g.uvCrota = float(uv_crota)
+    out = np.zeros((cc.shape[0], 4), np.float32)
+    n = _lib.load().ObitHostLoadCompsPoint(C.byref(g), cc.ctypes.data_as(C.POINTER(C.c_float)),
+                                           cc.shape[0], cc.shape[1], float(factor),
+                                           out.ctypes.data_as(C.POINTER(C.c_float)))
+    if n != cc.shape[0]:
+        raise _lib.HipError("ObitHostLoadCompsPoint failed")
+    return out
+
+
 class SkyModelDFT:
     """A component model applied to visibilities by the HIP DFT.
 

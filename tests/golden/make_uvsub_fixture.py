@@ -235,6 +235,9 @@ def main():
         cc=cc.astype(np.float32),
         freqs=freqs, f0=np.float64(f0), nif=nif, nchan=nchan, nstok=nstok,
         uv_centre=np.array([ra_c, dec_c]), rot_sign=sign, flux_scale=scale,
+        img_crval=np.array([float(img["CRVAL1"]), float(img["CRVAL2"])]),
+        img_crota=np.float32(float(img["CROTA2"])), uv_crota=np.float32(float(kw["6CROT6"])),
+        factor=np.float32(0.5), nfield=2,
         fit_rms=np.sqrt((rfin ** 2).sum() / npts), start_corr=corr, rows_kept=keep)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes)")
 
