@@ -9,16 +9,18 @@
 // the CPU's (the FP32 sincos of the facet phase shift is the same Cephes
 // sequence the CPU's SSE build evaluates).
 //
-// Accumulation: every facet-plane grid is cut into TU x TV-cell tiles.
-// A binning pass lists, for each tile, the (row, plane) tracks whose 7x7
-// footprints can touch it (the footprint of a track is bounded by the
-// samples at the plane's lowest and highest frequency, since cells are
-// monotonic in frequency).  Lists are cut into work items of at most
-// kChunk tracks; a wave takes an item, accumulates it in a private LDS
-// tile with lanes = the 49 footprint taps (no atomics, see k_grid_taps)
-// and adds the tile into the grid in HBM: with plain loads/stores when the
-// item is the tile's whole list, with device atomics when a long list was
-// split over several items.
+// Accumulation: every facet-plane grid is cut into TU x TV-cell tiles of
+// base cells (the first column and row of a sample's 7x7 footprint).  A
+// binning pass lists, for each tile, the (row, plane) tracks whose base
+// cells can fall in it (a track's base cells are bounded by the samples at
+// the plane's lowest and highest frequency, since cells are monotonic in
+// frequency).  Lists are cut into work items of at most kChunk tracks; a
+// wave takes an item, accumulates the samples whose base cell is in the
+// tile into a private LDS tile that carries a 7-cell halo to the right and
+// below, so every tap of every kept sample lands in it (lanes = the cells
+// of an 8x8 window round the footprint, see k_grid_taps), and adds the tile into the
+// grid in HBM: with plain loads/stores for the cells no other tile reaches
+// when the item is the tile's whole list, with device atomics otherwise.
 #include "common.h"
 #include "ObitCUDAGrid.h"
 #include "host.h"
@@ -38,7 +40,7 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 16;        // tile height in grid rows (LDS for 5 two-wave blocks per CU)
+constexpr int TV = 10;        // tile height in grid rows (LDS for 5 two-wave blocks per CU)
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
 
@@ -240,7 +242,8 @@ __global__ void k_zero2(int *a, int *b, long long n) {
        i += (long long)gridDim.x * blockDim.x) { a[i] = 0; b[i] = 0; }
 }
 
-// mode 0: count tracks per tile; mode 1: write row indices into lists.
+// mode 0: count tracks per tile of base cells; mode 1: write row indices
+// into lists.
 __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, int nf,
                       Common cm, int *counts, int *cursor, const int *offsets, int *lists,
                       int mode) {
@@ -260,8 +263,8 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
     cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
     cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
     const int ncol = f.lrow / 2;
-    int cA = min(iu0, iu1), cB = max(iu0, iu1) + cm.convWidth - 1;
-    int rA = min(iv0, iv1), rB = max(iv0, iv1) + cm.convWidth - 1;
+    int cA = min(iu0, iu1), cB = max(iu0, iu1);
+    int rA = min(iv0, iv1), rB = max(iv0, iv1);
     cA = max(cA, 0); rA = max(rA, 0);
     cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
     if (cA > cB || rA > rB) continue;
@@ -285,43 +288,57 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
 constexpr int kW = 7;          // GCF support in cells (checked on the host)
 constexpr int kChunk = 512;    // tracks per work item
 constexpr int kWaves = 2;      // waves per workgroup, each with a private tile
-// Tile row stride in floats, 7 mod 64: the 7x7 footprint of any base cell
-// then falls on 49 distinct banks (row tjv, column tju -> bank offset
-// 7*tjv + tju = the tap's lane number).
-constexpr int kS = TU + 7;
-constexpr int kPark = ((TV * kS + 63) / 64) * 64;  // 64 parking slots after the tile rows
-constexpr int kPlane = kPark + 64;                 // floats per re / im plane
+// A sample's footprint is accumulated through an 8x8 window of cells whose
+// origin is its base cell rounded down to even column and row, so a
+// window serves the 2x2 base cells of an aligned block.  Tile row stride
+// in floats 8 mod 64 and the im plane 16 mod 64 floats after the re plane:
+// the 32 (row, column, plane) cells one hand-over instruction touches fall
+// on distinct banks.
+constexpr int kWin = 8;
+constexpr int kS = TU + 8;
+constexpr int kTR = TV + kWin - 1;                 // tile rows with the halo below
+constexpr int kTC = TU + kWin - 1;                 // tile columns with the halo to the right
+constexpr int kPlane = ((kTR * kS + 47) / 64) * 64 + 16;  // floats per re / im plane
 constexpr int kStage = 24;     // floats per staged sample: 8 u-weights, 8 (re, im) v-products
-constexpr int kGroup = 4;      // staged samples read ahead per step
 constexpr int kHalf = 32;      // kept samples staged at a time
-constexpr int kStageN = kHalf + 2 * kGroup;  // stage entries per wave, with read-ahead slack
-static_assert(kHalf % (2 * kGroup) == 0, "the walk reads at most 2 groups past the last sample");
-constexpr int kNoCell = 0x7fffffff;
-static_assert(kS % 64 == 7 && kS >= TU, "tile stride must be 7 mod 64");
+constexpr int kStageN = kHalf; // stage entries per wave
+constexpr int kGrp = kHalf / 4;  // 4-sample groups per part (K of one matrix step)
+static_assert(kHalf % 4 == 0 && kHalf <= 64, "stage parts are whole groups of one pass");
+static_assert(kS % 64 == kWin && kS >= kTC, "tile stride must be 8 mod 64 and hold the halo");
+static_assert(kPlane % 64 == 16 && kPlane >= kTR * kS, "im plane offset must be 16 mod 64");
+static_assert(TU % 2 == 0 && TV % 2 == 0, "windows are aligned to even cells of the tile");
+static_assert(kW + 1 == kWin, "an 8x8 window holds a 7x7 footprint at either parity");
 static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 
 // Work items are (tile list, chunk of <= kChunk tracks).  Each wave takes
-// its own item and owns a private TV x TU tile (separate re / im planes)
-// in LDS, so no tile update needs an atomic.
+// its own item and owns a private (TV + 7) x (TU + 7) tile (separate re /
+// im planes) in LDS.
 //
 // The item's (track, channel) samples are visited 64 at a time, lanes =
 // samples in track-major, channel-minor order; the rows for the next 64
 // are fetched while the current 64 are gridded.  Each sample goes through
-// the CPU gridder's arithmetic (prep_vals); those whose 7x7 footprint
-// touches the tile are kept, in order, a 64-bit mask marks the kept
-// samples whose base cell differs from the previous one, and they pass
-// through a per-wave stage, kHalf at a time, as the seven u-weights and the
-// seven products v-weight x visibility.  Staged samples are walked with lanes = the 49
-// taps of the footprint: each tap lane keeps a running sum while the base
-// cell stays put and hands it to its tile cell when the mask says the
-// cell moves.  A hand-over is a read-modify-write whose write is deferred
-// to the next hand-over, so no step waits on an LDS read; all of one
-// hand-over's writes are issued before any of the next one's reads, which
-// keeps successive overlapping footprints exact.  Lanes without a tile
-// cell (the 15 spare lanes, taps off the tile's edge) park in a slot on
-// the bank their cell would have used.  A tile whose list fits one item
-// is added into the grid with plain stores; items of a longer list add
-// with device atomics.
+// the CPU gridder's arithmetic (prep_vals); those whose base cell lies in
+// the tile are kept, in order, a 64-bit mask marks the kept
+// samples whose window differs from the previous one's, and they pass
+// through a per-wave stage, kHalf at a time, as eight u-weights and eight
+// products v-weight x visibility already shifted into the window (the
+// seven taps at the base cell's parity, zero elsewhere).  While the window
+// stays put its 8x8 complex sums are a rank-k update, run on the matrix
+// cores 4 samples at a time: D[16][16] += A[16][4] B[4][16] with A's rows
+// the re and im parts of the v-products of the window's 8 rows, B's
+// columns the u-weights of its 8 columns (columns 8..15 zero) and K the
+// samples; a group of 4 that straddles a move is split by zeroing A
+// outside each segment.  The accumulator holds the window's running sums
+// and is handed to the tile when the mask says the window moves.  A
+// hand-over is a read-modify-write whose write is deferred to the next
+// hand-over, so no step waits on a fresh LDS read (LDS float atomics
+// measured 3x slower); all of one hand-over's writes are issued before any
+// of the next one's reads, which keeps successive overlapping windows
+// exact.
+// Cells that only this tile reaches are added into the
+// grid with plain stores when the item is the tile's whole list; the
+// halo, and every cell of an item of a longer list, add with device
+// atomics.
 __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, const FacetDev *facets,
                                                            const int *listFacet, const int2 *items,
                                                            long long nitems, Common cm,
@@ -336,12 +353,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     s_gcf8[i] = reinterpret_cast<const float4 *>(cm.gcf)[i];
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const bool tap = lane < kW * kW;
-  // spare lanes read a real staged entry; their sums are never handed over
-  const int tjv = tap ? lane / kW : 0, tju = tap ? lane - kW * (lane / kW) : 0;
-  const int offU = tju, offV = 8 + 2 * tjv;
-  const int laneOff = tap ? tjv * kS + tju : lane;  // == lane (mod 64) on every lane
-  const int tjuB = tap ? tju : (1 << 20);           // spare lanes never land in the tile
+  // matrix-step roles of this lane: operand column k (a sample of the
+  // group) and row / column m; accumulator rows 4k..4k+3 of column m
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int mk = lane >> 4, mm = lane & 15;
+  const bool mcol = mm < kWin;                          // a window column
+  const int offA = 8 + 2 * (mm & 7) + (mm >> 3);        // re (m < 8) or im of v-product m % 8
+  const int offB = mm & 7;                              // u-weight m (zeroed for m >= 8)
+  const int laneOff = (mk >> 1) * kPlane + 4 * (mk & 1) * kS + (mm & 7);
   float *re = reinterpret_cast<float *>(s_tile4[wv]);
   float *im = re + kPlane;
   float *stg = reinterpret_cast<float *>(s_stage[wv]);
@@ -349,13 +368,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
   for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
        w += (long long)gridDim.x * kWaves) {
     const int2 item = items[w];
-    const long long L = item.x;
+    const long long L = __builtin_amdgcn_readfirstlane(item.x);
     const int n = counts[L];
-    const int e0 = item.y * kChunk;
+    const int e0 = __builtin_amdgcn_readfirstlane(item.y) * kChunk;
     const int e1 = min(n, e0 + kChunk);
     const bool multi = n > kChunk;
     const int fi = listFacet[L];
-    const FacetDev &f = facets[fi];
+    // a register copy: the wave fences below would otherwise make every
+    // pass reload the descriptor from memory and wait on the row prefetch
+    const FacetDev f = facets[fi];
     const long long rel = L - f.listBase;
     const int per = f.ntu * f.ntv;
     const int p = (int)(rel / per);
@@ -368,68 +389,64 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     if (npc <= 0) continue;  // planes without channels have no tracks
     const int *pch = cm.pchan + (size_t)p * cm.maxpc;
     unsigned nStaged = 0, nMoves = 0;  // walk statistics of this item
-    int cur = kNoCell;               // packed base cell of the running sums
-    float ar = 0.0f, ai = 0.0f;      // running sums of this tap lane
-    int pend = kPark + lane;         // plane index of the deferred write
-    float pvr = 0.0f, pvi = 0.0f;    // plane values read for it
-    float par = 0.0f, pai = 0.0f;    // sums to add to them
+    // the sums start at zero for window 0, and the first deferred write
+    // stores zero into cells that are still zero
+    int cur = 0;                     // tile index of the window origin of the running sums
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};  // running sums: 4 window rows of column mm
+    int pend = laneOff;              // tile index of the deferred write (row 0 of 4)
+    f4 pv = {0.0f, 0.0f, 0.0f, 0.0f};   // tile values read for it
+    f4 pa = {0.0f, 0.0f, 0.0f, 0.0f};   // sums to add to them
     auto handover = [&]() {
-      re[pend] = pvr + par;
-      im[pend] = pvi + pai;
-      // kNoCell decodes to a column far right of any tile, so it parks
-      const int cs = (cur & 0xffff) - col0, rs = (cur >> 16) - row0;
-      const int idx = rs * kS + cs + laneOff;
-      const bool in = (unsigned)(cs + tjuB) < (unsigned)TU && (unsigned)(rs + tjv) < (unsigned)TV;
-      pend = in ? idx : (kPark | (idx & 63));
-      pvr = re[pend];
-      pvi = im[pend];
-      par = ar;
-      pai = ai;
+      if (mcol) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) re[pend + r * kS] = pv[r] + pa[r];
+        pend = cur + laneOff;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pv[r] = re[pend + r * kS];
+      }
+      pa = acc;
     };
     // slot (track e, channel index c) of this lane in the current pass and
-    // the next; a pass advances every lane by 64 slots
+    // the next; a pass advances every lane by 64 slots.  The row and
+    // channel of a slot are read two passes ahead and its data one pass
+    // ahead; slots past the item's end read its last track (always valid)
+    // so every pass issues the same loads.
     const int dq = 64 / npc, dr = 64 - dq * npc;
     int ea = e0 + lane / npc, ca = lane - (lane / npc) * npc;
     int eb = ea + dq, cb = ca + dr;
     if (cb >= npc) { cb -= npc; ++eb; }
-    float xu = 0.0f, xv = 0.0f, xw = 0.0f, xre = 0.0f, xim = 0.0f, xwt = 0.0f, xfs = 0.0f;
-    int xch = 0;
-    if (ea < e1) {
-      const float *row = vis + (long long)lst[ea] * cm.lenvis;
-      xch = pch[ca];
-      xfs = cm.freq[xch];
+    float xu, xv, xw, xre, xim, xwt, xfs;
+    int xch;
+    auto fetch = [&](int r, int ch) {
+      const float *row = vis + (long long)r * cm.lenvis;
+      xch = ch;
+      xfs = cm.freq[ch];
       xu = row[0]; xv = row[1]; xw = row[2];
-      const float *x = row + cm.nrparm + 3 * xch;
+      const float *x = row + cm.nrparm + 3 * ch;
       xre = x[0]; xim = x[1]; xwt = x[2];
-    }
-    long long rb = (eb < e1) ? lst[eb] : 0;
+    };
+    fetch(lst[min(ea, e1 - 1)], pch[ca]);
+    int rb = lst[min(eb, e1 - 1)], chb = pch[cb];
     const int npass = ((e1 - e0) * npc + 63) / 64;
     for (int pass = 0; pass < npass; ++pass) {
       const bool live = ea < e1;
       const float u0 = xu, v0 = xv, w0 = xw, re0 = xre, im0 = xim, wt0 = xwt, fs0 = xfs;
       const int ch0 = xch;
-      // fetch the next pass's rows and the index of the one after
+      // the slot after next: its row and channel; then the next pass's data
       int ec = eb + dq, cc2 = cb + dr;
       if (cc2 >= npc) { cc2 -= npc; ++ec; }
-      if (eb < e1) {
-        const float *row = vis + rb * cm.lenvis;
-        xch = pch[cb];
-        xfs = cm.freq[xch];
-        xu = row[0]; xv = row[1]; xw = row[2];
-        const float *x = row + cm.nrparm + 3 * xch;
-        xre = x[0]; xim = x[1]; xwt = x[2];
-      }
-      const long long rc = (ec < e1) ? lst[ec] : 0;
+      const int rc = lst[min(ec, e1 - 1)], chc = pch[cc2];
+      fetch(rb, chb);
       bool keep = false;
       Sample sm;
       if (live) {
         float uu, vv, ww;
         rotate3(f, u0, v0, w0, uu, vv, ww);
         sm = prep_vals(f, cm, uu, vv, ww, ch0, fs0, re0, im0, wt0);
-        keep = (sm.vr != 0.0f || sm.vi != 0.0f) && sm.iu + kW > col0 && sm.iu < col0 + TU &&
-               sm.iv + kW > row0 && sm.iv < row0 + TV;
+        keep = (sm.vr != 0.0f || sm.vi != 0.0f) && sm.iu >= col0 && sm.iu < col0 + TU &&
+               sm.iv >= row0 && sm.iv < row0 + TV;
       }
-      ea = eb; ca = cb; eb = ec; cb = cc2; rb = rc;
+      ea = eb; ca = cb; eb = ec; cb = cc2; rb = rc; chb = chc;
       const unsigned long long mask = __ballot(keep);
       if (mask == 0) continue;
       const int nv = __popcll(mask);
@@ -437,73 +454,82 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       if (keep) {
         pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                         __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
-        scell[pos] = (sm.iu & 0xffff) | (sm.iv << 16);
+        scell[pos] = ((sm.iv - row0) & ~1) * kS + ((sm.iu - col0) & ~1);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int cellq = scell[lane];  // lane q: base cell of kept sample q
+      const int cellq = scell[lane];  // lane q: tile index of kept sample q's window
       const int before = scell[lane > 0 ? lane - 1 : 0];
       const int cprev = (lane == 0) ? cur : before;
       const unsigned long long moves = __ballot(lane < nv && cellq != cprev);
       nStaged += nv;
       nMoves += __popcll(moves);
       // The kept samples go through the stage kHalf at a time (a smaller
-      // stage leaves LDS for more waves per CU).  Each part is walked
-      // kGroup samples at a time, reading the next group while the
-      // current one accumulates (two register sets in turn); the zeroed
-      // entries after a part's last sample make a partial group harmless.
+      // stage leaves LDS for more waves per CU).  Each lane reads its
+      // weight and product of every staged sample into registers at once,
+      // so the walk issues no LDS reads but the hand-overs' own: a
+      // hand-over then waits only on the previous hand-over's read.
+      // Entries past the part's last sample are zero, so the walk may run
+      // to the end of its kBlk-sample block.
       for (int lo = 0; lo < nv; lo += kHalf) {
         const int nh = min(kHalf, nv - lo);
         if (keep && pos >= lo && pos < lo + kHalf) {
           float4 *sp = reinterpret_cast<float4 *>(stg + (pos - lo) * kStage);
-          const float4 g0 = s_gcf8[2 * sm.itu], g1 = s_gcf8[2 * sm.itu + 1];
-          const float4 h0 = s_gcf8[2 * sm.itv], h1 = s_gcf8[2 * sm.itv + 1];
+          float4 g0 = s_gcf8[2 * sm.itu], g1 = s_gcf8[2 * sm.itu + 1];
+          float4 h0 = s_gcf8[2 * sm.itv], h1 = s_gcf8[2 * sm.itv + 1];
+          // an odd base column or row sits one cell into its window
+          if (sm.iu & 1) {
+            g1 = make_float4(g0.w, g1.x, g1.y, g1.z);
+            g0 = make_float4(0.0f, g0.x, g0.y, g0.z);
+          }
+          if (sm.iv & 1) {
+            h1 = make_float4(h0.w, h1.x, h1.y, h1.z);
+            h0 = make_float4(0.0f, h0.x, h0.y, h0.z);
+          }
           sp[0] = g0;
           sp[1] = g1;
           sp[2] = make_float4(h0.x * sm.vr, h0.x * sm.vi, h0.y * sm.vr, h0.y * sm.vi);
           sp[3] = make_float4(h0.z * sm.vr, h0.z * sm.vi, h0.w * sm.vr, h0.w * sm.vi);
           sp[4] = make_float4(h1.x * sm.vr, h1.x * sm.vi, h1.y * sm.vr, h1.y * sm.vi);
-          sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, 0.0f, 0.0f);
+          sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, h1.w * sm.vr, h1.w * sm.vi);
         }
-        if (lane < kGroup) {
-          float4 *sp = reinterpret_cast<float4 *>(stg + (nh + lane) * kStage);
+        if (lane >= nh && lane < kHalf) {
+          float4 *sp = reinterpret_cast<float4 *>(stg + lane * kStage);
 #pragma unroll
           for (int k = 0; k < kStage / 4; ++k) sp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        float wa[kGroup], wb[kGroup];
-        float2 aa[kGroup], ab[kGroup];
-        auto load = [&](float *w, float2 *a, int q0) {
+        float A[kGrp], B[kGrp];
 #pragma unroll
-          for (int j = 0; j < kGroup; ++j) {
-            w[j] = stg[(q0 + j) * kStage + offU];
-            a[j] = *reinterpret_cast<const float2 *>(stg + (q0 + j) * kStage + offV);
-          }
-        };
-        auto walk = [&](const float *w, const float2 *a, int q0) {
-          const unsigned long long mv = moves >> (lo + q0);
+        for (int g = 0; g < kGrp; ++g) {
+          A[g] = stg[(4 * g + mk) * kStage + offA];
+          B[g] = mcol ? stg[(4 * g + mk) * kStage + offB] : 0.0f;
+        }
+        const unsigned long long mv = moves >> lo;
 #pragma unroll
-          for (int j = 0; j < kGroup; ++j) {
-            if (mv & (1ull << j)) {
-              handover();
-              cur = __builtin_amdgcn_readlane(cellq, lo + q0 + j);
-              ar = 0.0f;
-              ai = 0.0f;
+        for (int g = 0; g < kGrp; ++g) {
+          if (4 * g >= nh) break;
+          unsigned mg = (unsigned)(mv >> (4 * g)) & 15u;  // moves at the group's samples
+          int s0 = 0;
+          while (mg) {
+            const int p1 = __builtin_ctz(mg);
+            if (p1 > s0) {
+              const unsigned seg = (1u << p1) - (1u << s0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g],
+                                                         acc, 0, 0, 0);
             }
-            ar = fmaf(w[j], a[j].x, ar);
-            ai = fmaf(w[j], a[j].y, ai);
+            handover();
+            cur = __builtin_amdgcn_readlane(cellq, lo + 4 * g + p1);
+            acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            s0 = p1;
+            mg &= mg - 1;
           }
-        };
-        load(wa, aa, 0);
-        for (int q = 0; q < nh; q += 2 * kGroup) {
-          load(wb, ab, q + kGroup);
-          walk(wa, aa, q);
-          if (q + kGroup >= nh) break;
-          load(wa, aa, q + 2 * kGroup);
-          walk(wb, ab, q + kGroup);
+          const unsigned seg = 16u - (1u << s0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g], acc,
+                                                     0, 0, 0);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -511,8 +537,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       }
     }
     handover();
-    re[pend] = pvr + par;
-    im[pend] = pvi + pai;
+    if (mcol) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) re[pend + r * kS] = pv[r] + pa[r];
+    }
     if (stats && lane == 0) {
       atomicAdd(&stats[0], (unsigned long long)nStaged);
       atomicAdd(&stats[1], (unsigned long long)nMoves);
@@ -522,19 +550,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float *g = f.grid + (size_t)p * f.sizeGrid;
     const int ncol = f.lrow / 2;
-    for (int i = lane; i < TV * TU; i += 64) {
-      const int rr = i / TU, cc = i - rr * TU;
+    for (int i = lane; i < kTR * kTC; i += 64) {
+      const int rr = i / kTC, cc = i - rr * kTC;
       const int gr = row0 + rr, gc = col0 + cc;
       if (gr >= f.ny || gc >= ncol) continue;
       const float vr = re[rr * kS + cc], vi = im[rr * kS + cc];
       if (vr == 0.0f && vi == 0.0f) continue;
       float *q = g + (size_t)gr * f.lrow + 2 * gc;
-      if (multi) {
-        unsafeAtomicAdd(q, vr);
-        unsafeAtomicAdd(q + 1, vi);
-      } else {
+      // the windows of the tiles to the left and above reach 6 cells into
+      // this one (their origins are even and below col0, row0)
+      const bool own = !multi && rr >= kWin - 1 && rr < TV && cc >= kWin - 1 && cc < TU;
+      if (own) {
         q[0] += vr;
         q[1] += vi;
+      } else {
+        unsafeAtomicAdd(q, vr);
+        unsafeAtomicAdd(q + 1, vi);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
