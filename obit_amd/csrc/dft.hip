@@ -11,10 +11,16 @@
 //     FP32 table), multiplied by nu/nu0 in FP64 and range-reduced to
 //     [-1/2, 1/2] turns in FP64;
 //   * one FP32 sincos for the phasor at the centre channel of the thread's
-//     channel chunk and one for the per-channel step exp(2 pi i psi dnu/nu0);
-//   * a first-order complex recurrence walks out from the centre in both
-//     directions (at most CPT/2 steps), so every channel costs one complex
-//     multiply and one complex multiply-add;
+//     channel chunk and one for half the per-channel step angle
+//     2 pi psi dnu/nu0, after the step is reduced to at most a quarter turn
+//     (the half turns taken out flip the sign of odd channel offsets);
+//   * the phasor walks out from the centre in both directions (at most
+//     CPT/2 steps), each step a rotation written as three shears (Paeth:
+//     x -= tan(d/2) y, y += sin(d) x, x -= tan(d/2) y), one FMA each, so
+//     every channel costs three FMAs for the step and two for the
+//     amplitude multiply-add; with |d| <= pi/2 the shear factors stay in
+//     [-1, 1] and the rounding of tan and sin perturbs the angle, not the
+//     modulus, by a relative few ulp;
 //   * sums run in FP32 over a component tile and are folded into FP64
 //     accumulators after every tile (two-level compensated summation).
 //
@@ -195,11 +201,20 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
         const double psi = fma(w, s_z[kk], fma(v, s_y[kk], u * s_x[kk]));
         double pm = psi * fm, pd = psi * fd;
         pm -= rint(pm);
-        pd -= rint(pd);
-        float sm, cm, sd, cd;
+        // step reduced to |pd| <= 1/4 turn; an odd count of half turns
+        // taken out makes every odd channel offset change sign
+        const double h2 = rint(2.0 * pd);
+        pd -= 0.5 * h2;
+        const bool flip = ((int)h2) & 1;
+        float sm, cm, sh, ch;
         sincos_turns<PREC == 2 ? 1 : 0>((float)pm, &sm, &cm);
-        sincos_turns<PREC == 0 ? 0 : 1>((float)pd, &sd, &cd);
+        sincos_turns<PREC == 0 ? 0 : 1>((float)(0.5 * pd), &sh, &ch);
+        // rotation by the step as three shears: tan and sin of the step
+        const float tau = sh * __builtin_amdgcn_rcpf(ch);
+        const float sig = 2.0f * sh * ch;
         const float a0 = s_a[kk];
+        const float sgn = flip ? -1.0f : 1.0f;
+        const float ae = a0, ao = a0 * sgn;  // amplitude at even / odd offsets
         float gf = a0, rf = 0.0f, rbk = 0.0f, qg = 0.0f;
         if (GAUSS) {
           const float G = fmaf(s_g[3 * kk + 2], uv, fmaf(s_g[3 * kk + 1], vv, s_g[3 * kk] * uu));
@@ -207,40 +222,45 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
           const float Gd = G * fdf;
           const bool live = e0 > -60.0f;
           gf = live ? (SPEC ? 1.0f : a0) * __expf(e0) : 0.0f;
-          rf = live ? __expf(Gd * (2.0f * fmf + fdf)) : 0.0f;
-          rbk = live ? __expf(Gd * (fdf - 2.0f * fmf)) : 0.0f;
+          // the per-step ratios carry the sign of the odd offsets
+          rf = live ? sgn * __expf(Gd * (2.0f * fmf + fdf)) : 0.0f;
+          rbk = live ? sgn * __expf(Gd * (fdf - 2.0f * fmf)) : 0.0f;
           qg = live ? __expf(2.0f * Gd * fdf) : 0.0f;
         }
         // forward from the centre channel
-        float zr = cm, zi = sm, g = gf, r = rf;
+        float x = cm, y = sm, g = gf, r = rf;
 #pragma unroll
         for (int j = CPT / 2; j < CPT; ++j) {
-          float amp = SPEC ? s_t[kk * CPT + j] : a0;
-          if (GAUSS) amp = SPEC ? amp * g : g;
-          accr[j] = fmaf(amp, zr, accr[j]);
-          acci[j] = fmaf(amp, zi, acci[j]);
+          const bool odd = (j - CPT / 2) & 1;
+          float amp;
+          if (GAUSS) amp = SPEC ? s_t[kk * CPT + j] * g : g;
+          else if (SPEC) amp = odd ? s_t[kk * CPT + j] * sgn : s_t[kk * CPT + j];
+          else amp = odd ? ao : ae;
+          accr[j] = fmaf(amp, x, accr[j]);
+          acci[j] = fmaf(amp, y, acci[j]);
           if (j + 1 < CPT) {
-            const float nr = zr * cd - zi * sd;
-            const float ni = zr * sd + zi * cd;
-            zr = nr; zi = ni;
+            const float x1 = fmaf(-tau, y, x);
+            y = fmaf(sig, x1, y);
+            x = fmaf(-tau, y, x1);
             if (GAUSS) { g *= r; r *= qg; }
           }
         }
-        // backward from the channel below the centre
-        float br = cm * cd + sm * sd, bi = sm * cd - cm * sd;
+        // backward from the centre channel
+        x = cm; y = sm;
         float gb = gf * rbk, rb2 = rbk * qg;
 #pragma unroll
         for (int j = CPT / 2 - 1; j >= 0; --j) {
-          float amp = SPEC ? s_t[kk * CPT + j] : a0;
-          if (GAUSS) amp = SPEC ? amp * gb : gb;
-          accr[j] = fmaf(amp, br, accr[j]);
-          acci[j] = fmaf(amp, bi, acci[j]);
-          if (j > 0) {
-            const float nr = br * cd + bi * sd;
-            const float ni = bi * cd - br * sd;
-            br = nr; bi = ni;
-            if (GAUSS) { gb *= rb2; rb2 *= qg; }
-          }
+          const bool odd = (CPT / 2 - j) & 1;
+          const float x1 = fmaf(tau, y, x);
+          y = fmaf(-sig, x1, y);
+          x = fmaf(tau, y, x1);
+          float amp;
+          if (GAUSS) amp = SPEC ? s_t[kk * CPT + j] * gb : gb;
+          else if (SPEC) amp = odd ? s_t[kk * CPT + j] * sgn : s_t[kk * CPT + j];
+          else amp = odd ? ao : ae;
+          accr[j] = fmaf(amp, x, accr[j]);
+          acci[j] = fmaf(amp, y, acci[j]);
+          if (GAUSS && j > 0) { gb *= rb2; rb2 *= qg; }
         }
       }
 #pragma unroll
