@@ -102,6 +102,22 @@ __device__ __forceinline__ void sincos_turns_poly(float t, float *s, float *c) {
   *c = (iq == 1 || iq == 2) ? -b : b;
 }
 
+// sin and cos of pi t for |t| <= 1/4 (an angle within pi/4): the Taylor
+// series of sincos_turns_poly without its quadrant reduction.
+__device__ __forceinline__ void sincos_halfturns_small(float t, float *s, float *c) {
+  const float x = t * 3.14159265358979323846f;
+  const float x2 = x * x;
+  float ps = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+  ps = fmaf(x2, ps, 8.33333333e-3f);
+  ps = fmaf(x2, ps, -1.66666667e-1f);
+  *s = fmaf(x * x2, ps, x);
+  float pc = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
+  pc = fmaf(x2, pc, -1.38888889e-3f);
+  pc = fmaf(x2, pc, 4.16666667e-2f);
+  pc = fmaf(x2, pc, -0.5f);
+  *c = fmaf(x2, pc, 1.0f);
+}
+
 template <int PREC>
 __device__ __forceinline__ void sincos_turns(float t, float *s, float *c) {
   if (PREC == 0) sincos_turns_hw(t, s, c);
@@ -208,13 +224,14 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
         const bool flip = ((int)h2) & 1;
         float sm, cm, sh, ch;
         sincos_turns<PREC == 2 ? 1 : 0>((float)pm, &sm, &cm);
-        sincos_turns<PREC == 0 ? 0 : 1>((float)(0.5 * pd), &sh, &ch);
+        if (PREC == 0) sincos_turns_hw((float)(0.5 * pd), &sh, &ch);
+        else sincos_halfturns_small((float)pd, &sh, &ch);
         // rotation by the step as three shears: tan and sin of the step
         const float tau = sh * __builtin_amdgcn_rcpf(ch);
         const float sig = 2.0f * sh * ch;
         const float a0 = s_a[kk];
         const float sgn = flip ? -1.0f : 1.0f;
-        const float ae = a0, ao = a0 * sgn;  // amplitude at even / odd offsets
+        const float ae = a0, ao = flip ? -a0 : a0;  // amplitude at even / odd offsets
         float gf = a0, rf = 0.0f, rbk = 0.0f, qg = 0.0f;
         if (GAUSS) {
           const float G = fmaf(s_g[3 * kk + 2], uv, fmaf(s_g[3 * kk + 1], vv, s_g[3 * kk] * uu));
