@@ -16,9 +16,10 @@
 // the plane's lowest and highest frequency, since cells are monotonic in
 // frequency).  Lists are cut into work items of at most kChunk tracks; a
 // wave takes an item, accumulates the samples whose base cell is in the
-// tile into a private LDS tile that carries a 7-cell halo to the right and
-// below, so every tap of every kept sample lands in it (lanes = the cells
-// of an 8x8 window round the footprint, see k_grid_taps), and adds the tile into the
+// tile into a private LDS tile that carries an 8-cell halo to the right and
+// a 6-cell halo below, so every tap of every kept sample lands in it
+// (lanes = the cells of a 16x8 window round the footprint, see
+// k_grid_taps), and adds the tile into the
 // grid in HBM: with plain loads/stores for the cells no other tile reaches
 // when the item is the tile's whole list, with device atomics otherwise.
 #include "common.h"
@@ -288,30 +289,28 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
 constexpr int kW = 7;          // GCF support in cells (checked on the host)
 constexpr int kChunk = 512;    // tracks per work item
 constexpr int kWaves = 2;      // waves per workgroup, each with a private tile
-// A sample's footprint is accumulated through an 8x8 window of cells whose
-// origin is its base cell rounded down to even column and row, so a
-// window serves the 2x2 base cells of an aligned block.  Tile row stride
-// in floats 8 mod 64 and the im plane 16 mod 64 floats after the re plane:
-// the 32 (row, column, plane) cells one hand-over instruction touches fall
-// on distinct banks.
-constexpr int kWin = 8;
-constexpr int kS = TU + 8;
-constexpr int kTR = TV + kWin - 1;                 // tile rows with the halo below
-constexpr int kTC = TU + kWin - 1;                 // tile columns with the halo to the right
-constexpr int kPlane = ((kTR * kS + 47) / 64) * 64 + 16;  // floats per re / im plane
-constexpr int kStage = 24;     // floats per staged sample: 8 u-weights, 8 (re, im) v-products
-constexpr int kHalf = 32;      // kept samples staged at a time
-constexpr int kStageN = kHalf; // stage entries per wave
-constexpr int kGrp = kHalf / 4;  // 4-sample groups per part (K of one matrix step)
-static_assert(kHalf % 4 == 0 && kHalf <= 64, "stage parts are whole groups of one pass");
-static_assert(kS % 64 == kWin && kS >= kTC, "tile stride must be 8 mod 64 and hold the halo");
-static_assert(kPlane % 64 == 16 && kPlane >= kTR * kS, "im plane offset must be 16 mod 64");
-static_assert(TU % 2 == 0 && TV % 2 == 0, "windows are aligned to even cells of the tile");
-static_assert(kW + 1 == kWin, "an 8x8 window holds a 7x7 footprint at either parity");
+// A sample's footprint is accumulated through a 16 (u) x 8 (v) window of
+// cells whose origin is its base cell rounded down to a multiple of 8 in u
+// and of 2 in v, so a window serves the 8x2 base cells of an aligned
+// block.  Tile row stride in floats 12 mod 16 and the im plane 32 mod 64
+// floats after the re plane: the 64 (row, column, plane) cells one
+// hand-over instruction touches fall on distinct banks.
+constexpr int kWinU = 16, kWinV = 8;              // window columns and rows
+constexpr int kBlkU = 8, kBlkV = 2;               // base cells a window serves
+constexpr int kS = TU + 12;
+constexpr int kTR = TV + kWinV - kBlkV;            // tile rows with the halo below
+constexpr int kTC = TU + kWinU - kBlkU;            // tile columns with the halo to the right
+constexpr int kPlane = ((kTR * kS + 31) / 64) * 64 + 32;  // floats per re / im plane
+constexpr int kHalf = 64;      // kept samples staged at a time (one pass)
+constexpr int kGrp = kHalf / 4;  // 4-sample groups of a pass (K of one matrix step)
+static_assert(kBlkU + kW - 1 <= kWinU && kBlkV + kW - 1 <= kWinV, "the window holds every footprint");
+static_assert(TU % kBlkU == 0 && TV % kBlkV == 0, "windows are aligned within the tile");
+static_assert(kS >= kTC && (4 * kS) % 64 == 48, "tile stride: room for the halo, 4 rows = 48 banks");
+static_assert(kPlane % 64 == 32 && kPlane >= kTR * kS, "im plane offset must be 32 mod 64");
 static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 
 // Work items are (tile list, chunk of <= kChunk tracks).  Each wave takes
-// its own item and owns a private (TV + 7) x (TU + 7) tile (separate re /
+// its own item and owns a private (TV + 6) x (TU + 8) tile (separate re /
 // im planes) in LDS.
 //
 // The item's (track, channel) samples are visited 64 at a time, lanes =
@@ -320,15 +319,15 @@ static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 // the CPU gridder's arithmetic (prep_vals); those whose base cell lies in
 // the tile are kept, in order, a 64-bit mask marks the kept
 // samples whose window differs from the previous one's, and they pass
-// through a per-wave stage, kHalf at a time, as eight u-weights and eight
-// products v-weight x visibility already shifted into the window (the
-// seven taps at the base cell's parity, zero elsewhere).  While the window
-// stays put its 8x8 complex sums are a rank-k update, run on the matrix
+// through a per-wave stage as six scalars: the base cell's offsets in its
+// window, the two GCF rows and the weighted visibility.  While the window
+// stays put its 16x8 complex sums are a rank-k update, run on the matrix
 // cores 4 samples at a time: D[16][16] += A[16][4] B[4][16] with A's rows
-// the re and im parts of the v-products of the window's 8 rows, B's
-// columns the u-weights of its 8 columns (columns 8..15 zero) and K the
-// samples; a group of 4 that straddles a move is split by zeroing A
-// outside each segment.  The accumulator holds the window's running sums
+// the re and im parts of the v-weight x visibility of the window's 8 rows,
+// B's columns the u-weights of its 16 columns and K the samples.  Each
+// lane gathers its own A and B element from the GCF table (a tap outside
+// the seven reads the table row's zero pad); a group of 4 that straddles
+// a move is split by zeroing A outside each segment.  The accumulator holds the window's running sums
 // and is handed to the tile when the mask says the window moves.  A
 // hand-over is a read-modify-write whose write is deferred to the next
 // hand-over, so no step waits on a fresh LDS read (LDS float atomics
@@ -339,14 +338,15 @@ static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 // grid with plain stores when the item is the tile's whole list; the
 // halo, and every cell of an item of a longer list, add with device
 // atomics.
-__global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, const FacetDev *facets,
+__global__ __launch_bounds__(64 * kWaves, 3) void k_grid_taps(const float *vis, const FacetDev *facets,
                                                            const int *listFacet, const int2 *items,
                                                            long long nitems, Common cm,
                                                            const int *counts, const int *offsets,
                                                            const int *lists,
                                                            unsigned long long *stats) {
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
-  __shared__ float4 s_stage[kWaves][kStageN * kStage / 4];
+  __shared__ int4 s_si[kWaves][kHalf];      // staged sample: du, 8 itu, dv, 8 itv
+  __shared__ float2 s_sv[kWaves][kHalf];    // staged sample: weighted visibility
   __shared__ int s_cell[kWaves][64];
   __shared__ float4 s_gcf8[101 * 2];  // row it: taps 0..6 of fractional offset it, then 0
   for (int i = threadIdx.x; i < 101 * 2; i += blockDim.x)
@@ -357,13 +357,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
   // group) and row / column m; accumulator rows 4k..4k+3 of column m
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int mk = lane >> 4, mm = lane & 15;
-  const bool mcol = mm < kWin;                          // a window column
-  const int offA = 8 + 2 * (mm & 7) + (mm >> 3);        // re (m < 8) or im of v-product m % 8
-  const int offB = mm & 7;                              // u-weight m (zeroed for m >= 8)
-  const int laneOff = (mk >> 1) * kPlane + 4 * (mk & 1) * kS + (mm & 7);
+  const int laneOff = (mk >> 1) * kPlane + 4 * (mk & 1) * kS + mm;
+  const float *gcf = reinterpret_cast<const float *>(s_gcf8);
   float *re = reinterpret_cast<float *>(s_tile4[wv]);
   float *im = re + kPlane;
-  float *stg = reinterpret_cast<float *>(s_stage[wv]);
+  int4 *si = s_si[wv];
+  float2 *sv = s_sv[wv];
   int *scell = s_cell[wv];
   for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
        w += (long long)gridDim.x * kWaves) {
@@ -397,13 +396,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     f4 pv = {0.0f, 0.0f, 0.0f, 0.0f};   // tile values read for it
     f4 pa = {0.0f, 0.0f, 0.0f, 0.0f};   // sums to add to them
     auto handover = [&]() {
-      if (mcol) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) re[pend + r * kS] = pv[r] + pa[r];
-        pend = cur + laneOff;
+      for (int r = 0; r < 4; ++r) re[pend + r * kS] = pv[r] + pa[r];
+      pend = cur + laneOff;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) pv[r] = re[pend + r * kS];
-      }
+      for (int r = 0; r < 4; ++r) pv[r] = re[pend + r * kS];
       pa = acc;
     };
     // slot (track e, channel index c) of this lane in the current pass and
@@ -454,7 +451,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       if (keep) {
         pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                         __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
-        scell[pos] = ((sm.iv - row0) & ~1) * kS + ((sm.iu - col0) & ~1);
+        scell[pos] = ((sm.iv - row0) & -kBlkV) * kS + ((sm.iu - col0) & -kBlkU);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -465,82 +462,59 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       const unsigned long long moves = __ballot(lane < nv && cellq != cprev);
       nStaged += nv;
       nMoves += __popcll(moves);
-      // The kept samples go through the stage kHalf at a time (a smaller
-      // stage leaves LDS for more waves per CU).  Each lane reads its
-      // weight and product of every staged sample into registers at once,
-      // so the walk issues no LDS reads but the hand-overs' own: a
-      // hand-over then waits only on the previous hand-over's read.
-      // Entries past the part's last sample are zero, so the walk may run
-      // to the end of its kBlk-sample block.
-      for (int lo = 0; lo < nv; lo += kHalf) {
-        const int nh = min(kHalf, nv - lo);
-        if (keep && pos >= lo && pos < lo + kHalf) {
-          float4 *sp = reinterpret_cast<float4 *>(stg + (pos - lo) * kStage);
-          float4 g0 = s_gcf8[2 * sm.itu], g1 = s_gcf8[2 * sm.itu + 1];
-          float4 h0 = s_gcf8[2 * sm.itv], h1 = s_gcf8[2 * sm.itv + 1];
-          // an odd base column or row sits one cell into its window
-          if (sm.iu & 1) {
-            g1 = make_float4(g0.w, g1.x, g1.y, g1.z);
-            g0 = make_float4(0.0f, g0.x, g0.y, g0.z);
-          }
-          if (sm.iv & 1) {
-            h1 = make_float4(h0.w, h1.x, h1.y, h1.z);
-            h0 = make_float4(0.0f, h0.x, h0.y, h0.z);
-          }
-          sp[0] = g0;
-          sp[1] = g1;
-          sp[2] = make_float4(h0.x * sm.vr, h0.x * sm.vi, h0.y * sm.vr, h0.y * sm.vi);
-          sp[3] = make_float4(h0.z * sm.vr, h0.z * sm.vi, h0.w * sm.vr, h0.w * sm.vi);
-          sp[4] = make_float4(h1.x * sm.vr, h1.x * sm.vi, h1.y * sm.vr, h1.y * sm.vi);
-          sp[5] = make_float4(h1.z * sm.vr, h1.z * sm.vi, h1.w * sm.vr, h1.w * sm.vi);
-        }
-        if (lane >= nh && lane < kHalf) {
-          float4 *sp = reinterpret_cast<float4 *>(stg + lane * kStage);
-#pragma unroll
-          for (int k = 0; k < kStage / 4; ++k) sp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        float A[kGrp], B[kGrp];
-#pragma unroll
-        for (int g = 0; g < kGrp; ++g) {
-          A[g] = stg[(4 * g + mk) * kStage + offA];
-          B[g] = mcol ? stg[(4 * g + mk) * kStage + offB] : 0.0f;
-        }
-        const unsigned long long mv = moves >> lo;
-#pragma unroll
-        for (int g = 0; g < kGrp; ++g) {
-          if (4 * g >= nh) break;
-          unsigned mg = (unsigned)(mv >> (4 * g)) & 15u;  // moves at the group's samples
-          int s0 = 0;
-          while (mg) {
-            const int p1 = __builtin_ctz(mg);
-            if (p1 > s0) {
-              const unsigned seg = (1u << p1) - (1u << s0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g],
-                                                         acc, 0, 0, 0);
-            }
-            handover();
-            cur = __builtin_amdgcn_readlane(cellq, lo + 4 * g + p1);
-            acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
-            s0 = p1;
-            mg &= mg - 1;
-          }
-          const unsigned seg = 16u - (1u << s0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g], acc,
-                                                     0, 0, 0);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // Stage the kept samples' scalars (zeros past the last one), read the
+      // pass's 16 groups back in matrix layout, gather the operands from
+      // the GCF table and walk the groups.
+      if (keep) {
+        const int cu = sm.iu - col0, cv = sm.iv - row0;
+        si[pos] = make_int4(cu & (kBlkU - 1), 8 * sm.itu, cv & (kBlkV - 1), 8 * sm.itv);
+        sv[pos] = make_float2(sm.vr, sm.vi);
       }
+      if (lane >= nv) {
+        si[lane] = make_int4(kWinU, 0, kWinV, 0);  // every tap on the zero pad
+        sv[lane] = make_float2(0.0f, 0.0f);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      float A[kGrp], B[kGrp];
+#pragma unroll
+      for (int g = 0; g < kGrp; ++g) {
+        const int4 q = si[4 * g + mk];
+        const float2 x = sv[4 * g + mk];
+        const unsigned tu = (unsigned)(mm - q.x), tv = (unsigned)((mm & 7) - q.z);
+        B[g] = gcf[q.y + (tu < 8u ? tu : 7u)];
+        A[g] = gcf[q.w + (tv < 8u ? tv : 7u)] * (mm < 8 ? x.x : x.y);
+      }
+#pragma unroll
+      for (int g = 0; g < kGrp; ++g) {
+        if (4 * g >= nv) break;
+        unsigned mg = (unsigned)(moves >> (4 * g)) & 15u;  // moves at the group's samples
+        int s0 = 0;
+        while (mg) {
+          const int p1 = __builtin_ctz(mg);
+          if (p1 > s0) {
+            const unsigned seg = (1u << p1) - (1u << s0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g], acc,
+                                                       0, 0, 0);
+          }
+          handover();
+          cur = __builtin_amdgcn_readlane(cellq, 4 * g + p1);
+          acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+          s0 = p1;
+          mg &= mg - 1;
+        }
+        const unsigned seg = 16u - (1u << s0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g], acc, 0,
+                                                   0, 0);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     handover();
-    if (mcol) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) re[pend + r * kS] = pv[r] + pa[r];
-    }
+    for (int r = 0; r < 4; ++r) re[pend + r * kS] = pv[r] + pa[r];
     if (stats && lane == 0) {
       atomicAdd(&stats[0], (unsigned long long)nStaged);
       atomicAdd(&stats[1], (unsigned long long)nMoves);
@@ -557,9 +531,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       const float vr = re[rr * kS + cc], vi = im[rr * kS + cc];
       if (vr == 0.0f && vi == 0.0f) continue;
       float *q = g + (size_t)gr * f.lrow + 2 * gc;
-      // the windows of the tiles to the left and above reach 6 cells into
-      // this one (their origins are even and below col0, row0)
-      const bool own = !multi && rr >= kWin - 1 && rr < TV && cc >= kWin - 1 && cc < TU;
+      // the windows of the tiles to the left and above reach kWin - kBlk
+      // cells into this one
+      const bool own = !multi && rr >= kWinV - kBlkV && rr < TV && cc >= kWinU - kBlkU && cc < TU;
       if (own) {
         q[0] += vr;
         q[1] += vi;
