@@ -41,7 +41,7 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 10;        // tile height in grid rows (LDS for 5 two-wave blocks per CU)
+constexpr int TV = 20;        // tile height in grid rows (LDS for 4 two-wave blocks per CU)
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
 
