@@ -203,6 +203,58 @@ def bench_fft2image(lib, gg, facets):
             "min_traffic_GBps": mib * 2**20 / (ms * 1e-3) / 1e9}
 
 
+def bench_stokes(lib, uv, reps=5, check_rows=2000, check=True):
+    """Stokes I records of workload 2's resident RR/LL rows
+    (ObitHIPStokesI, the gridder's input), timed with HIP events on the
+    default stream.  With check (the CPU leg), a seeded row sample is
+    compared with the oracle's restatement of ObitUVCalSelect (checker
+    only, outside the timing)."""
+    from obit_amd import _lib
+    from obit_amd.stokes import stokes_i_device
+    lout = uv.nrparm + 3 * uv.nif * uv.nchan
+    nin, nout = uv.buffer.nbytes, uv.nrow * lout * 4
+    d_in, d_out = C.c_void_p(), C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(d_in), nin), "malloc vis")
+    try:
+        _lib.check(lib.ObitHIPMalloc(C.byref(d_out), nout), "malloc I")
+        try:
+            _lib.check(lib.ObitHIPMemcpyH2D(d_in, _lib.ptr(uv.buffer), nin), "upload vis")
+            stokes_i_device(lib, d_in.value, d_out.value, uv)  # warm
+            lib.ObitHIPDeviceSynchronize()
+            e0, e1 = _events(lib)
+            lib.ObitCUDAEventRecord(e0, None)
+            for _ in range(reps):
+                stokes_i_device(lib, d_in.value, d_out.value, uv)
+            lib.ObitCUDAEventRecord(e1, None)
+            lib.ObitHIPDeviceSynchronize()
+            ms = lib.ObitHIPEventElapsedMs(e0, e1) / reps
+            rows = np.sort(np.random.default_rng(7).choice(uv.nrow, min(check_rows, uv.nrow), replace=False))
+            got = np.empty((rows.size, lout), np.float32)
+            for i, r in enumerate(rows):
+                _lib.check(lib.ObitHIPMemcpyD2H(got[i].ctypes.data, d_out.value + int(r) * lout * 4, lout * 4), "D2H")
+        finally:
+            lib.ObitHIPFree(d_out)
+    finally:
+        lib.ObitHIPFree(d_in)
+    exact = None
+    if check:
+        from oracle import oracle as orc
+        ref = orc.stokes_i(uv.buffer[rows], uv.nrparm, uv.nif, uv.nchan, uv.nstok, uv.incs, uv.incf,
+                           uv.incif, int(round(uv.crval_stokes)))
+        exact = bool(np.array_equal(got, ref, equal_nan=True))
+    # algorithmic bytes: the record's random parameters and first two
+    # correlations read, the Stokes I record written
+    nbytes = uv.nrow * 4 * (2 * uv.nrparm + uv.nif * uv.nchan * (6 + 3))
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"stokes I: {uv.nrow} rows in {ms:.2f} ms = {gbs:.0f} GB/s; sample of {rows.size} rows bit-identical "
+        f"to the oracle: {exact}")
+    return {"workload": "workload 2 RR/LL rows -> Stokes I records", "rows": uv.nrow, "ms": ms,
+            "rows_chan_per_s": uv.nrow * uv.nif * uv.nchan / (ms * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes},
+            "sample_rows_bit_identical": exact, "sample_rows": int(rows.size)}
+
+
 def bench_grid(lib, steps):
     from obit_amd import synth, _lib
     from obit_amd.gridder import GPUGrid, plane_of_channel
@@ -393,6 +445,7 @@ def main():
                              accuracy=not args.no_accuracy)
     vt = tuple(dft["variant"][k] for k in ("cpt", "tile", "wg_per_cu"))
     pcie = None if args.no_accuracy else bench_pcie(lib, uv, sky, vt)
+    stokes = None if args.no_grid else bench_stokes(lib, uv, check=not args.no_cpu)
     grid, gwork = (None, None) if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)
     if grid is not None and not args.no_cpu:
@@ -445,6 +498,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "gridding": grid,
+        "stokes_i": stokes,
     }
     print(json.dumps(line), flush=True)
 

@@ -117,6 +117,24 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
 int ObitHIPGridListStats(CUDAGridInfo *in, long iGPU, long long *stats);
 void *ObitHIPGridStream(CUDAGridInfo *in, long iGPU);
 
+/* ---- Stokes I records ---- */
+
+/* ObitUVCalSelect (src/ObitUVCalSelect.c:540-695) for a Stokes 'I'
+ * selection as ObitUVCalSelectIInit sets it up (:697-760), over a whole
+ * resident data set: d_in holds nvis rows of lenvis floats (nrparm random
+ * parameters, then correlations at float increments incs (Stokes), incf
+ * (channel), incif (IF), each [re, im, wt]); kstoke0 is the first
+ * correlation's Stokes code (-1 RR, -5 XX, 1 I; the CRVAL of the Stokes
+ * axis) and nstok their number; formalI requires both hands.  d_out
+ * receives nvis single-Stokes rows of nrparm + 3*nif*nchan floats (random
+ * parameters copied, then [IF][chan][re, im, wt]), bit-identical to the
+ * CPU's.  Returns OBIT_HIP_EINVAL, with ObitUVCalSelectIInit's message,
+ * when the data cannot give Stokes I.  Launched on stream (NULL: the
+ * default stream); returns without waiting. */
+int ObitHIPStokesI(const float *d_in, float *d_out, long long nvis, int lenvis, int nrparm,
+                   int nif, int nchan, int nstok, int incs, int incf, int incif, int kstoke0,
+                   int formalI, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

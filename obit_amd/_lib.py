@@ -147,6 +147,7 @@ _PROTOS = {
     "ObitHIPGridListStats": (_I, [_P, _L, _P]),
     "ObitHIPGridFFT2Image": (_I, [_P, _L, _I, _FP, _P]),
     "ObitHIPGridStream": (_P, [_P, _L]),
+    "ObitHIPStokesI": (_I, [_P, _P, _LL, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
     # host classes (csrc/host.h)
     "ObitHostDFTCreate": (_P, []),
     "ObitHostDFTInit": (_I, [_P, C.POINTER(HostUVDesc), C.POINTER(HostSkyModel), _I]),

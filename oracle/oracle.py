@@ -327,3 +327,59 @@ def fft2image(planes, convfn, is_beam, beam_norm=None, corr=None):
         norms = np.asarray(beam_norm, np.float64)
     out = img * (xc[None, :, None] / norms[:, None, None]) * yc[None, None, :]
     return out, norms
+
+
+# ---------------------------------------------------------------- Stokes I
+def stokes_i_setup(kstoke0, nstok, formal_i=False):
+    """ObitUVCalSelectIInit (src/ObitUVCalSelect.c:697-760): None when the
+    data cannot give Stokes I, else (second correlation's offset in Stokes
+    steps, factors (f1, f2), both hands required)."""
+    missing = (kstoke0 > 1) or (kstoke0 < -2)
+    if kstoke0 <= -5:
+        missing = kstoke0 < -6
+    if formal_i and kstoke0 < 0:
+        missing = missing or nstok < 2 or (kstoke0 != -1 and kstoke0 != -5)
+    if missing:
+        return None
+    if kstoke0 > 0 or nstok == 1:
+        return 0, (np.float32(1.0), np.float32(0.0)), False
+    return 1, (np.float32(0.5), np.float32(0.5)), bool(formal_i)
+
+
+def stokes_i(buffer, nrparm, nif, nchan, nstok, incs, incf, incif, kstoke0, formal_i=False):
+    """ObitUVCalSelect (src/ObitUVCalSelect.c:540-695) for a Stokes 'I'
+    selection of every channel and IF: rows of nrparm + 3 nif nchan
+    float32 (random parameters copied).  FP32 in the CPU's order."""
+    setup = stokes_i_setup(kstoke0, nstok, formal_i)
+    if setup is None:
+        raise ValueError("Stokes I not available")
+    j2, (f1, f2), both = setup
+    buf = np.asarray(buffer, np.float32)
+    nrow = buf.shape[0]
+    out = np.zeros((nrow, nrparm + 3 * nif * nchan), np.float32)
+    out[:, :nrparm] = buf[:, :nrparm]
+    half, zero, two = np.float32(0.5), np.float32(0.0), np.float32(2.0)
+    with np.errstate(invalid="ignore", over="ignore"):
+        for iif in range(nif):
+            for ch in range(nchan):
+                p = nrparm + iif * incif + ch * incf
+                q = p + j2 * incs
+                a, b = buf[:, p:p + 3], buf[:, q:q + 3]
+                # MAX (0.0, w) * 0.5: the comparison lets a NaN weight through
+                w1 = np.where(zero > a[:, 2], zero, a[:, 2]) * half
+                w2 = np.where(zero > b[:, 2], zero, b[:, 2]) * half
+                take = ~(both & ((w1 <= 0) | (w2 <= 0)))
+                wt = np.where(take, w1 + w2, zero).astype(np.float32)
+                live = take & (wt > 0)
+                x1 = np.where(w1 > 0, f1, zero).astype(np.float32)
+                x2 = np.where(w2 > 0, f2, zero).astype(np.float32)
+                vr = (x1 * a[:, 0] + x2 * b[:, 0]).astype(np.float32)
+                vi = (x1 * a[:, 1] + x2 * b[:, 1]).astype(np.float32)
+                one = (w1 <= 0) | (w2 <= 0)
+                vr = np.where(one, two * vr, vr)
+                vi = np.where(one, two * vi, vi)
+                o = nrparm + 3 * (iif * nchan + ch)
+                out[:, o] = np.where(live, vr, zero)
+                out[:, o + 1] = np.where(live, vi, zero)
+                out[:, o + 2] = wt
+    return out
