@@ -41,7 +41,7 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 20;        // tile height in grid rows (LDS for 4 two-wave blocks per CU)
+constexpr int TV = 20;        // tile height in grid rows (LDS for 2 four-wave blocks per CU)
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
 
@@ -288,7 +288,7 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
 
 constexpr int kW = 7;          // GCF support in cells (checked on the host)
 constexpr int kChunk = 512;    // tracks per work item
-constexpr int kWaves = 2;      // waves per workgroup, each with a private tile
+constexpr int kWaves = 4;      // waves per workgroup, each with a private tile
 // A sample's footprint is accumulated through a 16 (u) x 8 (v) window of
 // cells whose origin is its base cell rounded down to a multiple of 8 in u
 // and of 2 in v, so a window serves the 8x2 base cells of an aligned
@@ -303,6 +303,7 @@ constexpr int kTC = TU + kWinU - kBlkU;            // tile columns with the halo
 constexpr int kPlane = ((kTR * kS + 31) / 64) * 64 + 32;  // floats per re / im plane
 constexpr int kHalf = 64;      // kept samples staged at a time (one pass)
 constexpr int kGrp = kHalf / 4;  // 4-sample groups of a pass (K of one matrix step)
+constexpr int kGcfRow = 24;    // padded GCF row: 7 zeros, 7 taps, 10 zeros
 static_assert(kBlkU + kW - 1 <= kWinU && kBlkV + kW - 1 <= kWinV, "the window holds every footprint");
 static_assert(TU % kBlkU == 0 && TV % kBlkV == 0, "windows are aligned within the tile");
 static_assert(kS >= kTC && (4 * kS) % 64 == 48, "tile stride: room for the halo, 4 rows = 48 banks");
@@ -319,15 +320,17 @@ static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 // the CPU gridder's arithmetic (prep_vals); those whose base cell lies in
 // the tile are kept, in order, a 64-bit mask marks the kept
 // samples whose window differs from the previous one's, and they pass
-// through a per-wave stage as six scalars: the base cell's offsets in its
-// window, the two GCF rows and the weighted visibility.  While the window
+// through a per-wave stage as four scalars: where the sample's u and v
+// taps start in the padded GCF table (its GCF rows less its base cell's
+// offsets in the window) and the weighted visibility.  While the window
 // stays put its 16x8 complex sums are a rank-k update, run on the matrix
 // cores 4 samples at a time: D[16][16] += A[16][4] B[4][16] with A's rows
 // the re and im parts of the v-weight x visibility of the window's 8 rows,
 // B's columns the u-weights of its 16 columns and K the samples.  Each
-// lane gathers its own A and B element from the GCF table (a tap outside
-// the seven reads the table row's zero pad); a group of 4 that straddles
-// a move is split by zeroing A outside each segment.  The accumulator holds the window's running sums
+// lane gathers its own A and B element from the GCF table, whose rows are
+// padded with zeros so that any window column or row reads its tap or a
+// zero with no test; a group of 4 that straddles a move is split by
+// zeroing A outside each segment.  Four waves share a block and its table.  The accumulator holds the window's running sums
 // and is handed to the tile when the mask says the window moves.  A
 // hand-over is a read-modify-write whose write is deferred to the next
 // hand-over, so no step waits on a fresh LDS read (LDS float atomics
@@ -345,12 +348,16 @@ __global__ __launch_bounds__(64 * kWaves, 3) void k_grid_taps(const float *vis, 
                                                            const int *lists,
                                                            unsigned long long *stats) {
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
-  __shared__ int4 s_si[kWaves][kHalf];      // staged sample: du, 8 itu, dv, 8 itv
+  __shared__ int2 s_si[kWaves][kHalf];      // staged sample: u and v tap-row offsets in s_gcfp
   __shared__ float2 s_sv[kWaves][kHalf];    // staged sample: weighted visibility
   __shared__ int s_cell[kWaves][64];
-  __shared__ float4 s_gcf8[101 * 2];  // row it: taps 0..6 of fractional offset it, then 0
-  for (int i = threadIdx.x; i < 101 * 2; i += blockDim.x)
-    s_gcf8[i] = reinterpret_cast<const float4 *>(cm.gcf)[i];
+  // padded GCF rows: 7 zeros, the 7 taps of fractional offset it, 10 zeros,
+  // so every window column or row reads its tap (or a zero) with no test
+  __shared__ float s_gcfp[101 * kGcfRow];
+  for (int i = threadIdx.x; i < 101 * kGcfRow; i += blockDim.x) {
+    const int it = i / kGcfRow, j = i - it * kGcfRow - 7;
+    s_gcfp[i] = (j >= 0 && j < kW) ? cm.gcf[8 * it + j] : 0.0f;
+  }
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   // matrix-step roles of this lane: operand column k (a sample of the
@@ -358,10 +365,11 @@ __global__ __launch_bounds__(64 * kWaves, 3) void k_grid_taps(const float *vis, 
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int mk = lane >> 4, mm = lane & 15;
   const int laneOff = (mk >> 1) * kPlane + 4 * (mk & 1) * kS + mm;
-  const float *gcf = reinterpret_cast<const float *>(s_gcf8);
+  const float *gcf = s_gcfp;
+  const int mm7 = mm & 7;
   float *re = reinterpret_cast<float *>(s_tile4[wv]);
   float *im = re + kPlane;
-  int4 *si = s_si[wv];
+  int2 *si = s_si[wv];
   float2 *sv = s_sv[wv];
   int *scell = s_cell[wv];
   for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
@@ -467,11 +475,11 @@ __global__ __launch_bounds__(64 * kWaves, 3) void k_grid_taps(const float *vis, 
       // the GCF table and walk the groups.
       if (keep) {
         const int cu = sm.iu - col0, cv = sm.iv - row0;
-        si[pos] = make_int4(cu & (kBlkU - 1), 8 * sm.itu, cv & (kBlkV - 1), 8 * sm.itv);
+        si[pos] = make_int2(kGcfRow * sm.itu + 7 - (cu & (kBlkU - 1)), kGcfRow * sm.itv + 7 - (cv & (kBlkV - 1)));
         sv[pos] = make_float2(sm.vr, sm.vi);
       }
       if (lane >= nv) {
-        si[lane] = make_int4(kWinU, 0, kWinV, 0);  // every tap on the zero pad
+        si[lane] = make_int2(0, 0);  // any tap row: the visibility is zero
         sv[lane] = make_float2(0.0f, 0.0f);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -480,11 +488,10 @@ __global__ __launch_bounds__(64 * kWaves, 3) void k_grid_taps(const float *vis, 
       float A[kGrp], B[kGrp];
 #pragma unroll
       for (int g = 0; g < kGrp; ++g) {
-        const int4 q = si[4 * g + mk];
+        const int2 q = si[4 * g + mk];
         const float2 x = sv[4 * g + mk];
-        const unsigned tu = (unsigned)(mm - q.x), tv = (unsigned)((mm & 7) - q.z);
-        B[g] = gcf[q.y + (tu < 8u ? tu : 7u)];
-        A[g] = gcf[q.w + (tv < 8u ? tv : 7u)] * (mm < 8 ? x.x : x.y);
+        B[g] = gcf[q.x + mm];
+        A[g] = gcf[q.y + mm7] * (mm < 8 ? x.x : x.y);
       }
 #pragma unroll
       for (int g = 0; g < kGrp; ++g) {
