@@ -244,43 +244,44 @@ __global__ void k_zero2(int *a, int *b, long long n) {
 }
 
 // mode 0: count tracks per tile of base cells; mode 1: write row indices
-// into lists.
+// into lists.  One thread per (row, facet): the row is rotated once and
+// bounded for every plane.
 __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, int nf,
                       Common cm, int *counts, int *cursor, const int *offsets, int *lists,
                       int mode) {
-  const long long total = nrow * (long long)cm.nplane * nf;
+  const long long total = nrow * (long long)nf;
   for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
        it += (long long)gridDim.x * blockDim.x) {
     const long long r = it % nrow;
-    const long long fp = it / nrow;
-    const int p = (int)(fp % cm.nplane);
-    const int fi = (int)(fp / cm.nplane);
-    if (cm.pcount[p] <= 0) continue;
+    const int fi = (int)(it / nrow);
     const FacetDev &f = facets[fi];
     const float *row = vis + r * (long long)cm.lenvis;
     float uu, vv, ww;
     rotate_uvw(f, row, uu, vv, ww);
-    int iu0, iv0, iu1, iv1;
-    cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
-    cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
     const int ncol = f.lrow / 2;
-    int cA = min(iu0, iu1), cB = max(iu0, iu1);
-    int rA = min(iv0, iv1), rB = max(iv0, iv1);
-    cA = max(cA, 0); rA = max(rA, 0);
-    cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
-    if (cA > cB || rA > rB) continue;
-    const int tu0 = cA / TU, tu1 = cB / TU, tv0 = rA / TV, tv1 = rB / TV;
-    const long long base = f.listBase + (long long)p * f.ntu * f.ntv;
-    for (int tv = tv0; tv <= tv1; ++tv)
-      for (int tu = tu0; tu <= tu1; ++tu) {
-        const long long L = base + (long long)tv * f.ntu + tu;
-        if (mode == 0) {
-          atomicAdd(&counts[L], 1);
-        } else {
-          const int pos = atomicAdd(&cursor[L], 1);
-          lists[offsets[L] + pos] = (int)r;
+    for (int p = 0; p < cm.nplane; ++p) {
+      if (cm.pcount[p] <= 0) continue;
+      int iu0, iv0, iu1, iv1;
+      cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
+      cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
+      int cA = min(iu0, iu1), cB = max(iu0, iu1);
+      int rA = min(iv0, iv1), rB = max(iv0, iv1);
+      cA = max(cA, 0); rA = max(rA, 0);
+      cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
+      if (cA > cB || rA > rB) continue;
+      const int tu0 = cA / TU, tu1 = cB / TU, tv0 = rA / TV, tv1 = rB / TV;
+      const long long base = f.listBase + (long long)p * f.ntu * f.ntv;
+      for (int tv = tv0; tv <= tv1; ++tv)
+        for (int tu = tu0; tu <= tu1; ++tu) {
+          const long long L = base + (long long)tv * f.ntu + tu;
+          if (mode == 0) {
+            atomicAdd(&counts[L], 1);
+          } else {
+            const int pos = atomicAdd(&cursor[L], 1);
+            lists[offsets[L] + pos] = (int)r;
+          }
         }
-      }
+    }
   }
 }
 
@@ -869,7 +870,7 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
   hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_walk,
                      (int *)gs->d_walk + 2, 2LL);
   OBIT_HIP_TRY(hipGetLastError());
-  const long long ntr = nrow * (long long)gs->cm.nplane * nf;
+  const long long ntr = nrow * (long long)nf;
   const int nbB = (int)std::min<long long>((ntr + 255) / 256, (long long)gs->nCU * 32);
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 0);
