@@ -342,7 +342,7 @@ static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 // grid with plain stores when the item is the tile's whole list; the
 // halo, and every cell of an item of a longer list, add with device
 // atomics.
-__global__ __launch_bounds__(64 * kWaves, 3) void k_grid_taps(const float *vis, const FacetDev *facets,
+__global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, const FacetDev *facets,
                                                            const int *listFacet, const int2 *items,
                                                            long long nitems, Common cm,
                                                            const int *counts, const int *offsets,
