@@ -404,6 +404,131 @@ def read_traffic(name):
     return None
 
 
+def _virtual(nrow, lrec):
+    """A read-only (nrow, lrec) float32 array of one element: the host-side
+    descriptor of rows that live only on the device."""
+    return np.lib.stride_tricks.as_strided(np.zeros(1, np.float32), shape=(nrow, lrec), strides=(0, 0),
+                                           writeable=False)
+
+
+def bench_workload5(lib, sample_rows=32):
+    """Workload 5 at full size: 2016 baselines x 1,800 integrations x 1,024
+    channels (4 IF) RR/LL, 89 GB resident; one batched subtract of the
+    40-facet spectral model, Stokes I records, then one regrid into 40
+    facets x 8 coarse bins of 2048^2.  Visibilities are N(0,1) noise filled
+    on the device; a seeded row sample is kept on the host and checked
+    against the FP64 oracle (model and gridding) after the timed steps."""
+    from obit_amd import synth, _lib
+    from obit_amd.gridder import GPUGrid, plane_of_channel
+    from obit_amd.skymodel import SkyModelDFT
+    from obit_amd.stokes import stokes_i_device
+    rng = np.random.default_rng(synth.SEED0 + 5)
+    enu = synth.core_arm_array(rng)
+    f0 = 1.28e9
+    freqArr = synth.freq_table(f0, 4, 256, 1e6)
+    hdr = synth.uvw_tracks(enu, 34.08, 30.0, 1800, 8.0, f0).astype(np.float32)
+    _, sky, facets = synth.workload(5, rows=24)  # the tests' 40-facet model
+    nrow, nif, nchan, nstok = hdr.shape[0], 4, 256, 2
+    lrec = 5 + 3 * nif * nchan * nstok
+    lout = 5 + 3 * nif * nchan
+    uv = synth.UVData(buffer=_virtual(nrow, lrec), nchan=nchan, nif=nif, nstok=nstok, freq=f0,
+                      freqArr=freqArr, crval_stokes=-1.0)
+    log(f"workload 5: {nrow} rows x {nif * nchan} ch x {nstok} Stokes ({nrow * lrec * 4 / 1e9:.1f} GB), "
+        f"{sky.comps.shape[0]} spectral components in 40 facets")
+    d_vis, d_i = C.c_void_p(), C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(d_vis), nrow * lrec * 4), "malloc vis")
+    _lib.check(lib.ObitHIPMalloc(C.byref(d_i), nrow * lout * 4), "malloc Stokes I")
+    rows = np.sort(np.random.default_rng(9).choice(nrow, sample_rows, replace=False))
+    before = np.empty((sample_rows, lrec), np.float32)
+    try:
+        # N(0,1) visibilities, unit weights, generated and uploaded in chunks
+        step = 131072
+        chunk = np.empty((step, lrec), np.float32)
+        for b in range(0, nrow, step):
+            e = min(nrow, b + step)
+            c = chunk[:e - b]
+            c[:, :5] = hdr[b:e]
+            v = c[:, 5:].reshape(e - b, -1, 3)
+            v[:, :, :2] = rng.standard_normal((e - b, v.shape[1], 2), dtype=np.float32)
+            v[:, :, 2] = 1.0
+            sel = (rows >= b) & (rows < e)
+            before[sel] = c[rows[sel] - b]
+            _lib.check(lib.ObitHIPMemcpyH2D(C.c_void_p(d_vis.value + b * lrec * 4), c.ctypes.data,
+                                            c.nbytes), "upload")
+            if (b // step) % 7 == 0:
+                log(f"workload 5: uploaded {e} rows")
+        res = {"workload": "workload 5: 2016 baselines x 1,800 integrations x 1,024 ch RR/LL, 40 facets, "
+                           "2-term spectral model; subtract, Stokes I, regrid 40 x 8 x 2048^2",
+               "rows": nrow, "channels": nif * nchan, "components": int(sky.comps.shape[0])}
+        with SkyModelDFT(uv, sky, op="sub", nVisPIO=1024) as sm:
+            sm.attach_vis(d_vis.value, nrow)
+            res["dft_variant"] = sm.autotune(sample_rows=8192, reps=1)
+            log(f"workload 5: variant {res['dft_variant']}; one launch over every row follows")
+            st = sm.stream()
+            e0, e1 = _events(lib)
+            lib.ObitCUDAEventRecord(e0, st)
+            sm.calc_resident()
+            lib.ObitCUDAEventRecord(e1, st)
+            lib.ObitHIPDeviceSynchronize()
+            ms = lib.ObitHIPEventElapsedMs(e0, e1)
+        evals = nrow * nif * nchan * sky.comps.shape[0]
+        res["dft_ms"] = ms
+        res["dft_evals_per_s"] = evals / (ms * 1e-3)
+        log(f"workload 5: subtract {evals:.3e} evaluations in {ms / 1e3:.1f} s = {evals / (ms * 1e-3):.3e}/s")
+        after = np.empty_like(before)
+        for i, r in enumerate(rows):
+            _lib.check(lib.ObitHIPMemcpyD2H(after[i].ctypes.data, C.c_void_p(d_vis.value + int(r) * lrec * 4),
+                                            lrec * 4), "D2H")
+        e0, e1 = _events(lib)
+        lib.ObitCUDAEventRecord(e0, None)
+        stokes_i_device(lib, d_vis.value, d_i.value, uv)
+        lib.ObitCUDAEventRecord(e1, None)
+        lib.ObitHIPDeviceSynchronize()
+        res["stokes_i_ms"] = lib.ObitHIPEventElapsedMs(e0, e1)
+        lib.ObitHIPFree(d_vis)
+        d_vis = C.c_void_p()
+        uvI = synth.UVData(buffer=_virtual(nrow, lout), nchan=nchan, nif=nif, nstok=1, freq=f0,
+                           freqArr=freqArr, crval_stokes=1.0)
+        b, e = synth.coarse_bins(nif * nchan, 8)
+        poc = plane_of_channel(nif * nchan, b, e, cpu_parity=False)
+        with GPUGrid(uvI, facets, nplane=8, plane_of_chan=poc) as gg:
+            st = gg.stream()
+            e0, e1 = _events(lib)
+            lib.ObitCUDAEventRecord(e0, st)
+            gg.grid_device(d_i.value, nrow)
+            lib.ObitCUDAEventRecord(e1, st)
+            lib.ObitHIPDeviceSynchronize()
+            gms = lib.ObitHIPEventElapsedMs(e0, e1)
+            st8 = (C.c_longlong * 8)()
+            _lib.check(lib.ObitHIPGridListStats(gg.c, 0, st8), "list stats")
+    finally:
+        if d_vis.value:
+            lib.ObitHIPFree(d_vis)
+        lib.ObitHIPFree(d_i)
+    samples = nrow * nif * nchan * len(facets)
+    res["grid_ms"] = gms
+    res["grid_samples_per_s"] = samples / (gms * 1e-3)
+    res["grid_lists"] = dict(zip(("lists", "non_empty", "entries", "items", "longest", "split",
+                                  "tile_samples", "cell_moves"), list(st8)))
+    log(f"workload 5: Stokes I {res['stokes_i_ms']:.1f} ms; regrid {samples:.3e} samples in {gms:.0f} ms")
+    # the checker: the sampled rows' model (before - after) against the
+    # oracle's FP64 DFT, and their Stokes I records gridded on the device
+    # against the oracle's FP64 gridder
+    from oracle import oracle as orc
+    sub = synth.UVData(buffer=np.ascontiguousarray(before), nchan=nchan, nif=nif, nstok=nstok, freq=f0,
+                       freqArr=freqArr, crval_stokes=-1.0)
+    ref = orc.dft_model_fp64(sub, sky, nthreads=_cores()).reshape(sample_rows, nif * nchan)
+    d0 = before[:, 5:].reshape(sample_rows, nif * nchan, nstok, 3).astype(np.float64)
+    d1 = after[:, 5:].reshape(sample_rows, nif * nchan, nstok, 3).astype(np.float64)
+    got = (d0[..., 0, 0] - d1[..., 0, 0]) + 1j * (d0[..., 0, 1] - d1[..., 0, 1])
+    peak = np.abs(ref).max()
+    res["dft_err_max_vs_fp64_rel_peak"] = float(np.abs(got - ref).max() / peak)
+    res["dft_err_rms_vs_fp64_rel_peak"] = float(np.sqrt(np.mean(np.abs(got - ref) ** 2)) / peak)
+    res["check_rows"] = sample_rows
+    log(f"workload 5: sampled model max |dV|/peak {res['dft_err_max_vs_fp64_rel_peak']:.2e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +547,8 @@ def main():
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid-only", action="store_true", help="time the gridding workload alone")
+    ap.add_argument("--workload5", action="store_true",
+                    help="run workload 5 at full size (89 GB resident; about two minutes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -436,6 +563,9 @@ def main():
 
     if args.copy_mib:
         copy_calibration(lib, args.copy_mib)
+    if args.workload5:
+        print(json.dumps({"workload5": bench_workload5(lib)}), flush=True)
+        return
     if args.grid_only:
         grid, _ = bench_grid(lib, args.grid_steps)
         print(json.dumps({"gridding": grid}), flush=True)
