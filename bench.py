@@ -119,6 +119,51 @@ def bench_dft(lib, steps, warmup, sample_rows, variant=None, accuracy=True):
     return res, uv, sky
 
 
+def bench_batched(lib, steps=3, sample_rows=2048):
+    """Workload 3: the 40-facet model (point and Gaussian components, 10 to
+    20,000 per facet) subtracted from workload 1's visibilities in one
+    launch per step (graph replays, autotuned), and its error against the
+    FP64 comparator on a row sample."""
+    from obit_amd import synth
+    from obit_amd.skymodel import SkyModelDFT
+    uv, sky, _ = synth.workload(3)
+    evals = uv.nrow * uv.nif * uv.nchan * sky.comps.shape[0]
+    res = {"workload": "workload 3: 40 facets, 10-20,000 components each, 20 % Gaussian; one launch",
+           "rows": uv.nrow, "channels": uv.nif * uv.nchan, "components": int(sky.comps.shape[0]),
+           "gaussian_components": int((sky.comps[:, 4:7] != 0).any(axis=1).sum())}
+    with SkyModelDFT(uv, sky, op="sub") as sm:
+        sm.set_vis(uv.buffer)
+        res["variant"] = dict(zip(("cpt", "tile", "wg_per_cu"), sm.autotune(sample_rows=16384, reps=1)))
+        sm.graph_launch()
+        lib.ObitHIPDeviceSynchronize()
+        st = sm.stream()
+        e0, e1 = _events(lib)
+        lib.ObitCUDAEventRecord(e0, st)
+        for _ in range(steps):
+            sm.graph_launch()
+        lib.ObitCUDAEventRecord(e1, st)
+        lib.ObitHIPDeviceSynchronize()
+        ms = lib.ObitHIPEventElapsedMs(e0, e1) / steps
+    res["kernel_ms"] = ms
+    res["evals_per_s"] = evals / (ms * 1e-3)
+    rows = np.sort(np.random.default_rng(6).choice(uv.nrow, sample_rows, replace=False))
+    sub = uv.subset(rows)
+    out = {}
+    for kind in ("split", "fp64"):
+        with SkyModelDFT(sub, sky, op="repl") as s2:
+            s2.set_vis(sub.buffer)
+            s2.calc_resident() if kind == "split" else s2.calc_fp64()
+            d = s2.get_vis()[:, 5:].reshape(sub.nrow, sub.nif, sub.nchan, sub.nstok, 3)
+            out[kind] = d[..., 0, 0].astype(np.float64) + 1j * d[..., 0, 1]
+    peak = np.abs(out["fp64"]).max()
+    diff = np.abs(out["split"] - out["fp64"])
+    res["err_max_vs_fp64_rel_peak"] = float(diff.max() / peak)
+    res["err_rms_vs_fp64_rel_peak"] = float(np.sqrt(np.mean(diff ** 2)) / peak)
+    log(f"workload 3: {evals:.3e} evaluations in {ms:.1f} ms = {res['evals_per_s']:.3e}/s; "
+        f"max|dV|/peak {res['err_max_vs_fp64_rel_peak']:.2e}")
+    return res
+
+
 def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
     """Obit's per-buffer path: ObitCUDASkyModelDFTCalc on host buffers of
     nVisPIO rows (copy in to pinned memory, H2D, kernel, D2H, copy out),
@@ -576,6 +621,7 @@ def main():
     vt = tuple(dft["variant"][k] for k in ("cpt", "tile", "wg_per_cu"))
     pcie = None if args.no_accuracy else bench_pcie(lib, uv, sky, vt)
     stokes = None if args.no_grid else bench_stokes(lib, uv, check=not args.no_cpu)
+    batched = None if args.no_grid else bench_batched(lib)
     grid, gwork = (None, None) if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)
     if grid is not None and not args.no_cpu:
@@ -629,6 +675,7 @@ def main():
         "cpu_baseline": cpu,
         "gridding": grid,
         "stokes_i": stokes,
+        "batched_multifacet": batched,
     }
     print(json.dumps(line), flush=True)
 

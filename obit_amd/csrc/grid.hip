@@ -247,7 +247,7 @@ __global__ void k_zero2(int *a, int *b, long long n) {
 // into lists.  One thread per (row, facet): the row is rotated once and
 // bounded for every plane.
 __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, int nf,
-                      Common cm, int *counts, int *cursor, const int *offsets, int *lists,
+                      Common cm, int *counts, int *cursor, const long long *offsets, int *lists,
                       int mode) {
   const long long total = nrow * (long long)nf;
   for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
@@ -345,7 +345,7 @@ static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, const FacetDev *facets,
                                                            const int *listFacet, const int2 *items,
                                                            long long nitems, Common cm,
-                                                           const int *counts, const int *offsets,
+                                                           const int *counts, const long long *offsets,
                                                            const int *lists,
                                                            unsigned long long *stats) {
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
@@ -652,7 +652,8 @@ struct GpuState {
   std::vector<FacetDev> h_facets;
   int *d_listFacet = nullptr;
   long long nlist = 0;
-  int *d_counts = nullptr, *d_cursor = nullptr, *d_offsets = nullptr, *d_lists = nullptr;
+  int *d_counts = nullptr, *d_cursor = nullptr, *d_lists = nullptr;
+  long long *d_offsets = nullptr;  // first entry of every list (64-bit: passes may exceed 2^31 entries)
   long long listCap = 0;
   int2 *d_items = nullptr;
   long long itemCap = 0;
@@ -666,7 +667,8 @@ struct GpuState {
   size_t corrCap = 0;
   std::vector<float> h_corr;  // xCorr, yCorr of the last (nx, ny) (GridCorrFn)
   int corrNx = 0, corrNy = 0;
-  std::vector<int> h_counts, h_offsets;
+  std::vector<int> h_counts;
+  std::vector<long long> h_offsets;
   std::vector<int2> h_items;
   // tables
   float *d_gcf = nullptr, *d_freq = nullptr, *d_sig = nullptr, *d_pf = nullptr;
@@ -849,7 +851,7 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets);
   OBIT_HIP_TRY(hipMalloc(&gs->d_counts, sizeof(int) * (gs->nlist + 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_cursor, sizeof(int) * (gs->nlist + 1)));
-  OBIT_HIP_TRY(hipMalloc(&gs->d_offsets, sizeof(int) * (gs->nlist + 1)));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_offsets, sizeof(long long) * (gs->nlist + 1)));
   gs->ready = true;
   return kOK;
 }
@@ -885,14 +887,13 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
   gs->h_items.clear();
   long long total = 0;
   for (long long L = 0; L < nl; ++L) {
-    gs->h_offsets[L] = (int)total;
+    gs->h_offsets[L] = total;
     const int n = gs->h_counts[L];
     for (int c = 0; c * kChunk < n; ++c) gs->h_items.push_back(make_int2((int)L, c));
     total += n;
-    if (total > 0x7fffffffLL) return fail(kInvalid, "track lists exceed 2^31 entries");
   }
-  gs->h_offsets[nl] = (int)total;
-  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_offsets, gs->h_offsets.data(), sizeof(int) * (nl + 1),
+  gs->h_offsets[nl] = total;
+  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_offsets, gs->h_offsets.data(), sizeof(long long) * (nl + 1),
                               hipMemcpyHostToDevice, gs->stream));
   if ((long long)total > gs->listCap) {
     (void)hipFree(gs->d_lists);
