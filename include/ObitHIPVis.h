@@ -70,8 +70,9 @@ int ObitHIPSkyModelDFTCalcFP64(GPUInfo *gpuInfo);
 int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo);
 /* Kernel shape: channels per thread (16 or 32), component tile (32, 64
  * or 128), workgroups per CU of the persistent grid (1..8).  Autotune
- * times every shape on the first sampleRows resident rows and keeps the
- * fastest. */
+ * times every shape on a copy of the first resident rows and keeps the
+ * fastest: sampleRows rows, raised where needed so that the widest grid
+ * gets at least 8 rounds of work items, and capped at the resident rows. */
 int ObitHIPSkyModelDFTSetVariant(GPUInfo *gpuInfo, int cpt, int tile, int wgPerCU);
 int ObitHIPSkyModelDFTGetVariant(GPUInfo *gpuInfo, int *cpt, int *tile, int *wgPerCU);
 int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps);

@@ -33,6 +33,7 @@
 #include "ObitCUDASkyModel.h"
 #include "ObitCUDAUtil.h"
 #include "ObitHIPVis.h"
+#include "phasor.h"
 
 #include <algorithm>
 #include <cmath>
@@ -67,56 +68,6 @@ struct Desc {
   int op, doSwap;
   float stokFactor;
 };
-
-// ---- sin/cos of 2*pi*t ----
-
-// Hardware transcendental: v_sin_f32 / v_cos_f32 take revolutions.
-__device__ __forceinline__ void sincos_turns_hw(float t, float *s, float *c) {
-  *s = __builtin_amdgcn_sinf(t);
-  *c = __builtin_amdgcn_cosf(t);
-}
-
-// Quadrant reduction t = q/4 + r (exact for |t| <= 1/2), then Taylor
-// series of sin and cos in x = 2 pi r, |x| <= pi/4, truncated after x^9
-// and x^10 (truncation error < 2e-9), and a quarter-turn rotation.
-__device__ __forceinline__ void sincos_turns_poly(float t, float *s, float *c) {
-  const float q = rintf(4.0f * t);
-  const float r = fmaf(-0.25f, q, t);
-  const int iq = ((int)q) & 3;
-  const float x = r * 6.28318530717958647692f;
-  const float x2 = x * x;
-  float ps = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
-  ps = fmaf(x2, ps, 8.33333333e-3f);
-  ps = fmaf(x2, ps, -1.66666667e-1f);
-  const float sn = fmaf(x * x2, ps, x);
-  float pc = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
-  pc = fmaf(x2, pc, -1.38888889e-3f);
-  pc = fmaf(x2, pc, 4.16666667e-2f);
-  pc = fmaf(x2, pc, -0.5f);
-  const float cs = fmaf(x2, pc, 1.0f);
-  // rotate (cs, sn) by iq quarter turns
-  const bool odd = iq & 1;
-  const float a = odd ? cs : sn;   // sin of rotated angle before sign
-  const float b = odd ? sn : cs;   // cos of rotated angle before sign
-  *s = (iq == 2 || iq == 3) ? -a : a;
-  *c = (iq == 1 || iq == 2) ? -b : b;
-}
-
-// sin and cos of pi t for |t| <= 1/4 (an angle within pi/4): the Taylor
-// series of sincos_turns_poly without its quadrant reduction.
-__device__ __forceinline__ void sincos_halfturns_small(float t, float *s, float *c) {
-  const float x = t * 3.14159265358979323846f;
-  const float x2 = x * x;
-  float ps = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
-  ps = fmaf(x2, ps, 8.33333333e-3f);
-  ps = fmaf(x2, ps, -1.66666667e-1f);
-  *s = fmaf(x * x2, ps, x);
-  float pc = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
-  pc = fmaf(x2, pc, -1.38888889e-3f);
-  pc = fmaf(x2, pc, 4.16666667e-2f);
-  pc = fmaf(x2, pc, -0.5f);
-  *c = fmaf(x2, pc, 1.0f);
-}
 
 template <int PREC>
 __device__ __forceinline__ void sincos_turns(float t, float *s, float *c) {
@@ -983,8 +934,21 @@ int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps)
   State *st = find_state(gpuInfo);
   if (!st || !st->dvis) return obit::fail(obit::kState, "autotune needs resident visibilities");
   OBIT_HIP_TRY(hipSetDevice(st->device));
-  long long n = std::min(std::max(sampleRows, 1LL), st->nres);
   if (reps < 1) reps = 1;
+  // The sample must give the widest grid (4 workgroups per CU) at least 8
+  // rounds of items with the coarsest chunking (32 channels per thread);
+  // with fewer items than workgroups every grid width runs the same launch
+  // and the choice of width is noise.
+  long long n = std::max(sampleRows, 1LL);
+  {
+    obit::dft::Desc d1;
+    int rc1 = obit::dft::make_desc(st, gpuInfo, st->dvis, obit::dft::kRows, 32, d1);
+    if (rc1) return rc1;
+    const long long perBlk = std::max(d1.nitems, 1LL);
+    const long long blocks = (8LL * st->nCU * 4 + perBlk - 1) / perBlk;
+    n = std::max(n, blocks * obit::dft::kRows);
+  }
+  n = std::min(n, st->nres);
   const size_t bytes = (size_t)n * gpuInfo->lenvis * sizeof(float);
   float *tmp = nullptr;
   OBIT_HIP_TRY(hipMalloc(&tmp, bytes));
@@ -997,12 +961,13 @@ int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps)
   int bc = st->cpt, bt = st->tile, bw = st->wgPerCU;
   int rc = obit::kOK;
   for (int ci = 0; ci < 2 && !rc; ++ci) {
-    obit::dft::Desc d;
+    obit::dft::Desc d, dw;
     rc = obit::dft::make_desc(st, gpuInfo, tmp, n, cpts[ci], d);
+    if (!rc) rc = obit::dft::make_desc(st, gpuInfo, tmp, std::min(n, (long long)obit::dft::kRows), cpts[ci], dw);
     if (rc) break;
     for (int ti = 0; ti < 3 && !rc; ++ti) {
       for (int wi = 0; wi < 3 && !rc; ++wi) {
-        rc = obit::dft::launch_split(st, d, cpts[ci], tiles[ti], wgs[wi]);  // warm
+        rc = obit::dft::launch_split(st, dw, cpts[ci], tiles[ti], wgs[wi]);  // load the code object
         if (rc) break;
         (void)hipEventRecord(e0, st->stream);
         for (int r = 0; r < reps && !rc; ++r) rc = obit::dft::launch_split(st, d, cpts[ci], tiles[ti], wgs[wi]);
