@@ -34,6 +34,14 @@ class HostUVDesc(C.Structure):
     ]
 
 
+class BeamDesc(C.Structure):
+    """ObitHIPBeamDesc (include/ObitHIPBeam.h)."""
+    _fields_ = [(n, C.c_int) for n in (
+        "lrec", "nrparm", "ilocu", "ilocv", "ilocw", "ilocb", "iloca1", "iloca2",
+        "nif", "nchan", "incf", "incif", "ncorr", "ncomp", "compLen", "ntype", "nant",
+        "njchan", "stokType", "opType")] + [("stokFact", C.c_float * 4)]
+
+
 class HostSkyModel(C.Structure):
     _fields_ = [
         ("modType", C.c_int), ("nSpecTerm", C.c_int),
@@ -148,6 +156,12 @@ _PROTOS = {
     "ObitHIPGridFFT2Image": (_I, [_P, _L, _I, _FP, _P]),
     "ObitHIPGridStream": (_P, [_P, _L]),
     "ObitHIPStokesI": (_I, [_P, _P, _LL, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
+    # Jones-corrected DFT (ObitHIPBeam.h)
+    "ObitHIPBeamDFTCreate": (_I, [C.POINTER(_P), C.POINTER(BeamDesc), _P, _P, _P, _P, _P]),
+    "ObitHIPBeamDFTSetMod": (_I, [_P, _P]),
+    "ObitHIPBeamDFTSetJones": (_I, [_P, _P]),
+    "ObitHIPBeamDFTCalc": (_I, [_P, _P, _LL, _F]),
+    "ObitHIPBeamDFTDestroy": (_I, [_P]),
     # host classes (csrc/host.h)
     "ObitHostDFTCreate": (_P, []),
     "ObitHostDFTInit": (_I, [_P, C.POINTER(HostUVDesc), C.POINTER(HostSkyModel), _I]),

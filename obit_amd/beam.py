@@ -1,0 +1,104 @@
+"""Jones-corrected (antenna-beam-corrected) DFT sky model for linear feeds
+through ObitHIPBeamDFT* (include/ObitHIPBeam.h).
+
+Drives the layer the way Obit's beam-model host class drives its CUDA
+layer (reference src/ObitGPUBeamModel.c: Init, SetMod, Calc per
+parallactic angle, Shutdown; src/ObitCUDABeamModel.cu:1368-1437), with the
+Jones table handed in instead of interpolated from beam images.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+OPS = {"sub": 0, "div": 1, "repl": 2}
+
+
+def freq_lines(uv):
+    """nu/nu0 of channel 0 of each IF and the per-channel step, from the
+    IF-major frequency table (channels are evenly spaced within an IF)."""
+    f = np.asarray(uv.freqArr, np.float64).reshape(uv.nif, uv.nchan) / uv.freq
+    step = (f[:, 1] - f[:, 0]) if uv.nchan > 1 else np.zeros(uv.nif)
+    lin = f[:, :1] + np.arange(uv.nchan)[None, :] * step[:, None]
+    if np.max(np.abs(lin - f)) > 1e-9 * np.max(np.abs(f)):
+        raise ValueError("channel frequencies are not evenly spaced within an IF")
+    return np.ascontiguousarray(f[:, 0]), np.ascontiguousarray(step)
+
+
+class BeamDFT:
+    """One data layout, component table and Jones table on the device."""
+
+    def __init__(self, uv, comps, ant_type, vis_jchan, njchan, ntype=None, stok_type=1, op="sub",
+                 stok_fact=(1.0, 1.0, 1.0, 1.0), ncorr=None, stream=None):
+        self.lib = _lib.load()
+        self.uv = uv
+        comps = np.ascontiguousarray(comps, np.float32)
+        ant_type = np.ascontiguousarray(ant_type, np.int32)
+        vis_jchan = np.ascontiguousarray(vis_jchan, np.int32)
+        d = _lib.BeamDesc()
+        d.lrec, d.nrparm = uv.lrec, uv.nrparm
+        d.ilocu, d.ilocv, d.ilocw = uv.ilocu, uv.ilocv, uv.ilocw
+        d.ilocb, d.iloca1, d.iloca2 = uv.ilocb, -1, -1
+        d.nif, d.nchan, d.incf, d.incif = uv.nif, uv.nchan, uv.incf, uv.incif
+        d.ncorr = int(ncorr if ncorr is not None else uv.nstok)
+        d.ncomp, d.compLen = comps.shape
+        d.ntype = int(ntype if ntype is not None else ant_type.max() + 1)
+        d.nant = len(ant_type)
+        d.njchan = int(njchan)
+        d.stokType, d.opType = int(stok_type), OPS[op]
+        for i in range(4):
+            d.stokFact[i] = float(stok_fact[i])
+        self.desc = d
+        f0, df = freq_lines(uv)
+        self.h = C.c_void_p()
+        _lib.check(self.lib.ObitHIPBeamDFTCreate(C.byref(self.h), C.byref(d), f0.ctypes.data, df.ctypes.data,
+                                                 ant_type.ctypes.data, vis_jchan.ctypes.data, stream),
+                   "ObitHIPBeamDFTCreate")
+        _lib.check(self.lib.ObitHIPBeamDFTSetMod(self.h, comps.ctypes.data), "ObitHIPBeamDFTSetMod")
+
+    def set_jones(self, jones=None):
+        """jones: complex [ntype, ncomp, njchan, 2, 2] (or None: unit)."""
+        if jones is None:
+            _lib.check(self.lib.ObitHIPBeamDFTSetJones(self.h, None), "ObitHIPBeamDFTSetJones")
+            return
+        j = np.asarray(jones, np.complex64)
+        flat = np.ascontiguousarray(j.reshape(j.shape[:3] + (4,)).view(np.float32))
+        _lib.check(self.lib.ObitHIPBeamDFTSetJones(self.h, flat.ctypes.data), "ObitHIPBeamDFTSetJones")
+
+    def calc_device(self, d_vis, nrow, par_ang=0.0):
+        _lib.check(self.lib.ObitHIPBeamDFTCalc(self.h, C.c_void_p(d_vis), nrow, float(par_ang)),
+                   "ObitHIPBeamDFTCalc")
+
+    def calc(self, buffer, par_ang=0.0):
+        """Apply the model to a copy of host rows; returns the new rows."""
+        buf = np.ascontiguousarray(buffer, np.float32).copy()
+        d = C.c_void_p()
+        _lib.check(self.lib.ObitHIPMalloc(C.byref(d), max(buf.nbytes, 4)), "malloc")
+        try:
+            _lib.check(self.lib.ObitHIPMemcpyH2D(d, buf.ctypes.data, buf.nbytes), "H2D")
+            self.calc_device(d.value, buf.shape[0], par_ang)
+            _lib.check(self.lib.ObitHIPDeviceSynchronize(), "sync")
+            _lib.check(self.lib.ObitHIPMemcpyD2H(buf.ctypes.data, d, buf.nbytes), "D2H")
+        finally:
+            self.lib.ObitHIPFree(d)
+        return buf
+
+    def close(self):
+        if self.h:
+            self.lib.ObitHIPBeamDFTDestroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

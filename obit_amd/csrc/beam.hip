@@ -1,0 +1,555 @@
+// beam.hip -- Jones-corrected (antenna-beam-corrected) DFT sky model for
+// linear feeds (ObitHIPBeam.h).
+//
+// Replaces the evaluation of reference src/ObitCUDABeamModel.cu
+// (beamPointKernel :139-310, beamGaussKernel :687-858), whose inner loop
+// forms, per (visibility, channel, component), the four products of the
+// baseline's two Jones matrices with the Stokes coherency
+// (include/CUDALinCorSum.h).  Those products depend on the component, the
+// two antenna types, the beam channel and the parallactic angle, not on
+// the visibility, so they are formed once per call:
+//   k_beam_coh  C[pair][comp][jchan] = J1 P J2^H   (pair = type1 * ntype + type2)
+// and the DFT is then a sum of complex multiply-adds of the component's
+// phasor into each correlation:
+//   k_beam_dft  S_e(row, chan) = sum_k a_k exp(2 pi i psi_k nu/nu0) C_e
+// with the phasor walk of the split DFT (FP64 phase and range reduction,
+// FP32 sincos at the centre of an 8-channel chunk, the step as three
+// shears) and FP32 tile sums folded into FP64.
+//
+// Work decomposition: a workgroup is 256 consecutive rows x one (IF,
+// 8-channel chunk); each thread owns one row.  When the chunk's channels
+// share one beam channel (the usual case: beam channels are coarse), the
+// tile's products for every antenna-type pair are staged in LDS and each
+// thread reads its pair's once per component; otherwise each channel
+// reads its product from the table in global memory.
+#include "common.h"
+#include "ObitHIPBeam.h"
+#include "phasor.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace obit {
+namespace beam {
+
+constexpr int kRows = 256;     // rows (threads) per workgroup
+constexpr int kCPT = 8;        // channels per thread
+constexpr int kTile = 64;      // components per LDS tile
+constexpr int kMaxPairs = 16;  // type pairs staged in LDS (4 antenna types)
+constexpr int kMaxTypes = 16;
+
+struct Desc {
+  float *vis;
+  long long nrow;
+  int lrec, nrparm, ilocu, ilocv, ilocw, ilocb, iloca1, iloca2;
+  int nchan, incf, incif;
+  int nchunk, nrowblk;
+  long long nitems;
+  int ncomp, ntype, nant, njchan, npair;
+  const double *fIF, *dfIF;
+  const double *cx, *cy, *cz;  // turns per wavelength at nu0
+  const float *camp;
+  const float *cg;             // Gaussian coefficients [3][ncomp]
+  const int *antType, *jmap;
+  const float2 *coh;           // [pair][comp][jchan][4]: XX, XY, YX, YY
+  int op;
+  float sf[4];                 // XX, XY, YX, YY
+};
+
+// One product per (pair, component, beam channel): beam1 * P * beam2^H,
+// in the CPU's order (model * beam2^H, then beam1 * that;
+// src/ObitSkyModelVMBeamMF.c:2752-2755).
+__global__ __launch_bounds__(256) void k_beam_coh(const float *__restrict__ jones,
+                                                  float2 *__restrict__ coh, int ntype, int ncomp,
+                                                  int njchan, int stokType, float c, float s) {
+  const long long n = (long long)ntype * ntype * ncomp * njchan;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int jc = (int)(i % njchan);
+  const int k = (int)((i / njchan) % ncomp);
+  const int p = (int)(i / ((long long)njchan * ncomp));
+  const int t1 = p / ntype, t2 = p - t1 * ntype;
+  const float *A = jones + (((long long)t1 * ncomp + k) * njchan + jc) * 8;
+  const float *B = jones + (((long long)t2 * ncomp + k) * njchan + jc) * 8;
+  // coherency P of the model's Stokes parameter, linear feeds
+  float Pr[4] = {1.0f, 0.0f, 0.0f, 1.0f}, Pi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (stokType == 2) {
+    Pr[0] = c; Pr[1] = -s; Pr[2] = -s; Pr[3] = -c;
+  } else if (stokType == 3) {
+    Pr[0] = s; Pr[1] = c; Pr[2] = c; Pr[3] = -s;
+  } else if (stokType == 4) {
+    Pr[0] = 0.0f; Pr[1] = 0.0f; Pr[2] = 0.0f; Pr[3] = 0.0f;
+    Pi[1] = 1.0f; Pi[2] = -1.0f;
+  }
+  // T = P * B^H: T[r][q] = sum_m P[r][m] conj(B[q][m])
+  float Tr[4], Ti[4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      float re = 0.0f, im = 0.0f;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const float br = B[2 * (2 * q + m)], bi = -B[2 * (2 * q + m) + 1];
+        const float pr = Pr[2 * r + m], pi = Pi[2 * r + m];
+        re += pr * br - pi * bi;
+        im += pr * bi + pi * br;
+      }
+      Tr[2 * r + q] = re;
+      Ti[2 * r + q] = im;
+    }
+  }
+  // M = A * T
+  float2 *out = coh + i * 4;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      float re = 0.0f, im = 0.0f;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const float ar = A[2 * (2 * r + m)], ai = A[2 * (2 * r + m) + 1];
+        re += ar * Tr[2 * m + q] - ai * Ti[2 * m + q];
+        im += ar * Ti[2 * m + q] + ai * Tr[2 * m + q];
+      }
+      out[2 * r + q] = make_float2(re, im);
+    }
+  }
+}
+
+// Antenna numbers (0-rel) from BASELINE = 256 a1 + a2 (+ subarray
+// fraction) or from ANTENNA1/ANTENNA2 (the reference's GetAnts,
+// ObitCUDABeamModel.cu:82-98).
+__device__ __forceinline__ void row_ants(const Desc &d, const float *rp, int *a1, int *a2) {
+  if (d.ilocb >= 0) {
+    const int cb = (int)rp[d.ilocb];
+    const int x = (int)((double)cb / 256.0 + 0.001);
+    *a1 = x - 1;
+    *a2 = (int)((double)(cb - x * 256) + 0.001) - 1;
+  } else {
+    *a1 = (int)(rp[d.iloca1] + 0.5f) - 1;
+    *a2 = (int)(rp[d.iloca2] + 0.5f) - 1;
+  }
+}
+
+// Accumulate v = (vr, vi) times product m into (ar, ai).
+__device__ __forceinline__ void cmac(float vr, float vi, float2 m, float &ar, float &ai) {
+  ar = fmaf(vr, m.x, fmaf(-vi, m.y, ar));
+  ai = fmaf(vr, m.y, fmaf(vi, m.x, ai));
+}
+
+// NC correlations: 2 (XX, YY: products 0 and 3) or 4 (XX, XY, YX, YY).
+template <int NC>
+__device__ __forceinline__ int prod_index(int e) {
+  return NC == 2 ? 3 * e : e;
+}
+
+template <int NC, bool GAUSS>
+__global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
+  __shared__ double s_x[kTile], s_y[kTile], s_z[kTile];
+  __shared__ float s_a[kTile];
+  __shared__ float s_g[GAUSS ? 3 * kTile : 1];
+  __shared__ float2 s_m[kTile * kMaxPairs * NC];
+  const int tid = threadIdx.x;
+
+  for (long long item = blockIdx.x; item < d.nitems; item += gridDim.x) {
+    const int rb = (int)(item % d.nrowblk);
+    const int ic = (int)(item / d.nrowblk);
+    const int iIF = ic / d.nchunk;
+    const int c0 = (ic - iIF * d.nchunk) * kCPT;
+    const long long row = (long long)rb * kRows + tid;
+    const bool valid = row < d.nrow;
+    const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
+    int a1 = 0, a2 = 0;
+    if (valid) row_ants(d, rp, &a1, &a2);
+    const bool live = valid && a1 >= 0 && a1 < d.nant && a2 >= 0 && a2 < d.nant;
+    const int pair = live ? d.antType[a1] * d.ntype + d.antType[a2] : 0;
+    const float uf = live ? rp[d.ilocu] : 0.0f;
+    const float vf = live ? rp[d.ilocv] : 0.0f;
+    const float wf = live ? rp[d.ilocw] : 0.0f;
+    const double u = uf, v = vf, w = wf;
+    // beam channel of each channel of the chunk (the same for every lane)
+    const int *jm = d.jmap + (long long)iIF * d.nchan;
+    int jcs[kCPT];
+    bool uniform = true;
+#pragma unroll
+    for (int j = 0; j < kCPT; ++j) {
+      jcs[j] = __builtin_amdgcn_readfirstlane(jm[min(c0 + j, d.nchan - 1)]);
+      uniform = uniform && (jcs[j] == jcs[0]);
+    }
+    const bool useLds = uniform && d.npair <= kMaxPairs;
+    const double fd = d.dfIF[iIF];
+    const double fm = d.fIF[iIF] + (double)(c0 + kCPT / 2) * fd;
+    const float fmf = (float)fm, fdf = (float)fd;
+    const float uu = uf * uf, vv = vf * vf, uv = uf * vf;
+
+    float accr[NC][kCPT], acci[NC][kCPT];
+    double outr[NC][kCPT], outi[NC][kCPT];
+#pragma unroll
+    for (int e = 0; e < NC; ++e)
+#pragma unroll
+      for (int j = 0; j < kCPT; ++j) {
+        accr[e][j] = acci[e][j] = 0.0f;
+        outr[e][j] = outi[e][j] = 0.0;
+      }
+
+    for (int k0 = 0; k0 < d.ncomp; k0 += kTile) {
+      __syncthreads();
+      for (int t = tid; t < kTile; t += kRows) {
+        const int k = k0 + t;
+        const bool ok = k < d.ncomp;
+        s_x[t] = ok ? d.cx[k] : 0.0;
+        s_y[t] = ok ? d.cy[k] : 0.0;
+        s_z[t] = ok ? d.cz[k] : 0.0;
+        s_a[t] = ok ? d.camp[k] : 0.0f;
+        if (GAUSS) {
+          s_g[3 * t + 0] = ok ? d.cg[k] : 0.0f;
+          s_g[3 * t + 1] = ok ? d.cg[d.ncomp + k] : 0.0f;
+          s_g[3 * t + 2] = ok ? d.cg[2 * d.ncomp + k] : 0.0f;
+        }
+      }
+      if (useLds) {
+        const int per = d.npair * NC;
+        for (int t = tid; t < kTile * per; t += kRows) {
+          const int kk = t / per, r = t - kk * per;
+          const int p = r / NC, e = r - p * NC;
+          const int k = k0 + kk;
+          s_m[t] = (k < d.ncomp)
+                       ? d.coh[(((long long)p * d.ncomp + k) * d.njchan + jcs[0]) * 4 + prod_index<NC>(e)]
+                       : make_float2(0.0f, 0.0f);
+        }
+      }
+      __syncthreads();
+      const int kn = min(kTile, d.ncomp - k0);
+      for (int kk = 0; kk < kn; ++kk) {
+        const double psi = fma(w, s_z[kk], fma(v, s_y[kk], u * s_x[kk]));
+        double pm = psi * fm, pd = psi * fd;
+        pm -= rint(pm);
+        // step reduced to |pd| <= 1/4 turn; an odd count of half turns
+        // taken out makes every odd channel offset change sign
+        const double h2 = rint(2.0 * pd);
+        pd -= 0.5 * h2;
+        const bool flip = ((int)h2) & 1;
+        float sm, cm, sh, ch;
+        sincos_turns_poly((float)pm, &sm, &cm);
+        sincos_halfturns_small((float)pd, &sh, &ch);
+        const float tau = sh * __builtin_amdgcn_rcpf(ch);
+        const float sig = 2.0f * sh * ch;
+        const float a0 = s_a[kk];
+        const float sgn = flip ? -1.0f : 1.0f;
+        const float ao = flip ? -a0 : a0;
+        float gf = a0, rf = 0.0f, rbk = 0.0f, qg = 0.0f;
+        if (GAUSS) {
+          const float G = fmaf(s_g[3 * kk + 2], uv, fmaf(s_g[3 * kk + 1], vv, s_g[3 * kk] * uu));
+          const float e0 = G * fmf * fmf;
+          const float Gd = G * fdf;
+          const bool on = e0 > -60.0f;
+          gf = on ? a0 * __expf(e0) : 0.0f;
+          rf = on ? sgn * __expf(Gd * (2.0f * fmf + fdf)) : 0.0f;
+          rbk = on ? sgn * __expf(Gd * (fdf - 2.0f * fmf)) : 0.0f;
+          qg = on ? __expf(2.0f * Gd * fdf) : 0.0f;
+        }
+        // this component's products: from LDS once, or per channel
+        float mur[NC], mui[NC];
+        const float2 *mg = d.coh + ((long long)pair * d.ncomp + k0 + kk) * d.njchan * 4;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+          const float2 m = useLds ? s_m[(kk * d.npair + pair) * NC + e] : make_float2(0.0f, 0.0f);
+          mur[e] = m.x;
+          mui[e] = m.y;
+        }
+        // forward from the centre channel
+        float x = cm, y = sm, g = gf, r = rf;
+#pragma unroll
+        for (int j = kCPT / 2; j < kCPT; ++j) {
+          const bool odd = (j - kCPT / 2) & 1;
+          {
+            const float amp = GAUSS ? g : (odd ? ao : a0);
+            const float vr = amp * x, vi = amp * y;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+              const float2 m = useLds ? make_float2(mur[e], mui[e]) : mg[jcs[j] * 4 + prod_index<NC>(e)];
+              cmac(vr, vi, m, accr[e][j], acci[e][j]);
+            }
+          }
+          if (j + 1 < kCPT) {
+            const float x1 = fmaf(-tau, y, x);
+            y = fmaf(sig, x1, y);
+            x = fmaf(-tau, y, x1);
+            if (GAUSS) { g *= r; r *= qg; }
+          }
+        }
+        // backward from the centre channel
+        x = cm; y = sm;
+        float gb = gf * rbk, rb2 = rbk * qg;
+#pragma unroll
+        for (int j = kCPT / 2 - 1; j >= 0; --j) {
+          const bool odd = (kCPT / 2 - j) & 1;
+          const float x1 = fmaf(tau, y, x);
+          y = fmaf(-sig, x1, y);
+          x = fmaf(tau, y, x1);
+          {
+            const float amp = GAUSS ? gb : (odd ? ao : a0);
+            const float vr = amp * x, vi = amp * y;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+              const float2 m = useLds ? make_float2(mur[e], mui[e]) : mg[jcs[j] * 4 + prod_index<NC>(e)];
+              cmac(vr, vi, m, accr[e][j], acci[e][j]);
+            }
+          }
+          if (GAUSS && j > 0) { gb *= rb2; rb2 *= qg; }
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < NC; ++e)
+#pragma unroll
+        for (int j = 0; j < kCPT; ++j) {
+          outr[e][j] += (double)accr[e][j];
+          outi[e][j] += (double)acci[e][j];
+          accr[e][j] = acci[e][j] = 0.0f;
+        }
+    }
+
+    if (!live) continue;
+    // apply, following src/ObitSkyModelVMBeamMF.c:3711-3800
+    float *base = d.vis + row * (long long)d.lrec + d.nrparm + (long long)iIF * d.incif;
+#pragma unroll
+    for (int j = 0; j < kCPT; ++j) {
+      if (c0 + j >= d.nchan) break;
+      float *p = base + (long long)(c0 + j) * d.incf;
+      const bool repl = d.op == ObitHIPBeamOpRepl;
+      bool allBad = (p[2] <= 0.0f) && (p[5] <= 0.0f);
+      if (NC == 4) allBad = allBad && (p[8] <= 0.0f) && (p[11] <= 0.0f);
+      if (allBad && !repl) continue;
+      // XX and YY (products 0 and 3)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = (NC == 2) ? h : 3 * h;
+        const int prod = 3 * h;
+        float mr = (float)outr[e][j] * d.sf[prod], mi = (float)outi[e][j] * d.sf[prod];
+        float *q = p + 3 * h;
+        if (!(q[2] > 0.0f || repl)) continue;
+        if (d.op == ObitHIPBeamOpDiv) {
+          float wm = mr * mr + mi * mi;
+          mr = mr / wm;
+          mi = mi / wm;
+          wm = sqrtf(wm);
+          const float re = q[0], im = q[1];
+          q[0] = mr * re + mi * im;
+          q[1] = mr * im - mi * re;
+          q[2] = q[2] * wm;
+        } else if (repl) {
+          q[0] = mr;
+          q[1] = mi;
+        } else {
+          q[0] -= mr;
+          q[1] -= mi;
+        }
+      }
+      // XY (product 1) at +6 and YX (product 2) at +9: not when dividing
+      if (NC == 4 && d.op != ObitHIPBeamOpDiv) {
+#pragma unroll
+        for (int h = 1; h <= 2; ++h) {
+          const float mr = (float)outr[h][j] * d.sf[h], mi = (float)outi[h][j] * d.sf[h];
+          float *q = p + 3 + 3 * h;
+          if (!(q[2] > 0.0f || repl)) continue;
+          if (repl) {
+            q[0] = mr;
+            q[1] = mi;
+          } else {
+            q[0] -= mr;
+            q[1] -= mi;
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace beam
+}  // namespace obit
+
+struct ObitHIPBeam {
+  ObitHIPBeamDesc desc;
+  hipStream_t stream = nullptr;
+  int ncu = 0;
+  bool gauss = false, anyFlux = false, haveMod = false;
+  double *d_freq = nullptr;   // fIF[nif], dfIF[nif]
+  int *d_antType = nullptr, *d_jmap = nullptr;
+  double *d_pos = nullptr;    // cx, cy, cz [3][ncomp]
+  float *d_amp = nullptr, *d_g = nullptr;
+  float *d_jones = nullptr;
+  float2 *d_coh = nullptr;
+};
+
+namespace {
+
+void release(ObitHIPBeam *h) {
+  if (!h) return;
+  void *ptrs[] = {h->d_freq, h->d_antType, h->d_jmap, h->d_pos, h->d_amp, h->d_g, h->d_jones, h->d_coh};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ObitHIPBeamDFTCreate(ObitHIPBeam **out, const ObitHIPBeamDesc *desc, const double *fIF,
+                         const double *dfIF, const int *antType, const int *visJChan,
+                         void *stream) {
+  using namespace obit;
+  if (!out || !desc || !fIF || !dfIF || !antType || !visJChan)
+    return fail(kInvalid, "ObitHIPBeamDFTCreate: null argument");
+  *out = nullptr;
+  const ObitHIPBeamDesc &d = *desc;
+  if (d.lrec < 1 || d.nrparm < 0 || d.nif < 1 || d.nchan < 1 || d.incf < 1 || d.incif < 1)
+    return fail(kInvalid, "ObitHIPBeamDFTCreate: bad record layout");
+  if (d.ncorr != 2 && d.ncorr != 4) return fail(kInvalid, "ObitHIPBeamDFTCreate: ncorr must be 2 or 4");
+  const long long last = (long long)d.nrparm + (long long)(d.nif - 1) * d.incif +
+                         (long long)(d.nchan - 1) * d.incf + 3LL * (d.ncorr - 1) + 2;
+  if (last >= d.lrec) return fail(kInvalid, "ObitHIPBeamDFTCreate: correlations exceed the %d-float row", d.lrec);
+  auto inrow = [&](int i) { return i >= 0 && i < d.lrec; };
+  if (!inrow(d.ilocu) || !inrow(d.ilocv) || !inrow(d.ilocw))
+    return fail(kInvalid, "ObitHIPBeamDFTCreate: u, v, w outside the row");
+  if (d.ilocb < 0 && (!inrow(d.iloca1) || !inrow(d.iloca2)))
+    return fail(kInvalid, "ObitHIPBeamDFTCreate: no baseline or antenna parameters");
+  if (d.ilocb >= d.lrec) return fail(kInvalid, "ObitHIPBeamDFTCreate: baseline outside the row");
+  if (d.ncomp < 0 || d.compLen < 4) return fail(kInvalid, "ObitHIPBeamDFTCreate: bad component table shape");
+  if (d.ntype < 1 || d.ntype > beam::kMaxTypes || d.nant < 1 || d.njchan < 1)
+    return fail(kInvalid, "ObitHIPBeamDFTCreate: bad antenna types / beam channels");
+  if (d.stokType < 1 || d.stokType > 4) return fail(kInvalid, "ObitHIPBeamDFTCreate: stokType must be 1..4");
+  if (d.opType < 0 || d.opType > 2) return fail(kInvalid, "ObitHIPBeamDFTCreate: bad opType");
+  for (int i = 0; i < d.nant; ++i)
+    if (antType[i] < 0 || antType[i] >= d.ntype)
+      return fail(kInvalid, "ObitHIPBeamDFTCreate: antenna %d has type %d of %d", i, antType[i], d.ntype);
+  for (long long i = 0; i < (long long)d.nif * d.nchan; ++i)
+    if (visJChan[i] < 0 || visJChan[i] >= d.njchan)
+      return fail(kInvalid, "ObitHIPBeamDFTCreate: beam channel %d of entry %lld outside 0..%d", visJChan[i], i,
+                  d.njchan - 1);
+  ObitHIPBeam *h = new ObitHIPBeam();
+  h->desc = d;
+  h->stream = (hipStream_t)stream;
+  int rc = kOK;
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    OBIT_HIP_TRY(hipMalloc(dst, std::max<size_t>(bytes, 16)));
+    if (src && bytes) OBIT_HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return kOK;
+  };
+  std::vector<double> f(2 * (size_t)d.nif);
+  for (int i = 0; i < d.nif; ++i) { f[i] = fIF[i]; f[d.nif + i] = dfIF[i]; }
+  int dev = 0;
+  if (!rc) rc = (hipGetDevice(&dev) == hipSuccess) ? kOK : fail(kHipError, "hipGetDevice failed");
+  if (!rc && hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    rc = fail(kHipError, "hipDeviceGetAttribute failed");
+  if (!rc) rc = up((void **)&h->d_freq, f.data(), f.size() * sizeof(double));
+  if (!rc) rc = up((void **)&h->d_antType, antType, sizeof(int) * d.nant);
+  if (!rc) rc = up((void **)&h->d_jmap, visJChan, sizeof(int) * (size_t)d.nif * d.nchan);
+  const size_t nc = (size_t)std::max(d.ncomp, 1);
+  if (!rc) rc = up((void **)&h->d_pos, nullptr, 3 * nc * sizeof(double));
+  if (!rc) rc = up((void **)&h->d_amp, nullptr, nc * sizeof(float));
+  if (!rc) rc = up((void **)&h->d_g, nullptr, 3 * nc * sizeof(float));
+  if (!rc) rc = up((void **)&h->d_jones, nullptr, (size_t)d.ntype * nc * d.njchan * 8 * sizeof(float));
+  if (!rc) rc = up((void **)&h->d_coh, nullptr, (size_t)d.ntype * d.ntype * nc * d.njchan * 4 * sizeof(float2));
+  if (rc) { release(h); return rc; }
+  *out = h;
+  return ObitHIPBeamDFTSetJones(h, nullptr);
+}
+
+int ObitHIPBeamDFTSetMod(ObitHIPBeam *h, const float *comps) {
+  using namespace obit;
+  if (!h || (!comps && h->desc.ncomp > 0)) return fail(kInvalid, "ObitHIPBeamDFTSetMod: null argument");
+  const int n = h->desc.ncomp, L = h->desc.compLen;
+  std::vector<double> pos(3 * (size_t)std::max(n, 1));
+  std::vector<float> amp(std::max(n, 1)), g(3 * (size_t)std::max(n, 1));
+  bool gauss = false, any = false;
+  const double inv2pi = 1.0 / (2.0 * M_PI);
+  for (int k = 0; k < n; ++k) {
+    const float *c = comps + (size_t)k * L;
+    amp[k] = c[0];
+    any = any || c[0] != 0.0f;
+    for (int a = 0; a < 3; ++a) pos[(size_t)a * n + k] = (double)c[1 + a] * inv2pi;
+    for (int a = 0; a < 3; ++a) {
+      g[(size_t)a * n + k] = L >= 7 ? c[4 + a] : 0.0f;
+      gauss = gauss || g[(size_t)a * n + k] != 0.0f;
+    }
+  }
+  if (n > 0) {
+    OBIT_HIP_TRY(hipMemcpy(h->d_pos, pos.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    OBIT_HIP_TRY(hipMemcpy(h->d_amp, amp.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    OBIT_HIP_TRY(hipMemcpy(h->d_g, g.data(), 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  h->gauss = gauss;
+  h->anyFlux = any;
+  h->haveMod = true;
+  return kOK;
+}
+
+int ObitHIPBeamDFTSetJones(ObitHIPBeam *h, const float *jones) {
+  using namespace obit;
+  if (!h) return fail(kInvalid, "ObitHIPBeamDFTSetJones: null handle");
+  const size_t n = (size_t)h->desc.ntype * std::max(h->desc.ncomp, 1) * h->desc.njchan;
+  if (jones) {
+    OBIT_HIP_TRY(hipMemcpy(h->d_jones, jones, n * 8 * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> unit(n * 8, 0.0f);
+    for (size_t i = 0; i < n; ++i) { unit[8 * i] = 1.0f; unit[8 * i + 6] = 1.0f; }
+    OBIT_HIP_TRY(hipMemcpy(h->d_jones, unit.data(), n * 8 * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return kOK;
+}
+
+int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAng) {
+  using namespace obit;
+  if (!h || (!d_vis && nrow > 0) || nrow < 0) return fail(kInvalid, "ObitHIPBeamDFTCalc: bad arguments");
+  if (!h->haveMod) return fail(kState, "ObitHIPBeamDFTCalc: no component table (ObitHIPBeamDFTSetMod)");
+  const ObitHIPBeamDesc &D = h->desc;
+  // no component with flux: the data are left as they are (the reference
+  // kernels' gotSome)
+  if (nrow == 0 || D.ncomp == 0 || !h->anyFlux) return kOK;
+  const double c2 = std::cos(2.0 * (double)parAng), s2 = std::sin(2.0 * (double)parAng);
+  const long long ncoh = (long long)D.ntype * D.ntype * D.ncomp * D.njchan;
+  hipLaunchKernelGGL(beam::k_beam_coh, dim3((unsigned)((ncoh + 255) / 256)), dim3(256), 0, h->stream,
+                     h->d_jones, h->d_coh, D.ntype, D.ncomp, D.njchan, D.stokType, (float)c2, (float)s2);
+  OBIT_HIP_TRY(hipGetLastError());
+  beam::Desc d;
+  d.vis = d_vis;
+  d.nrow = nrow;
+  d.lrec = D.lrec; d.nrparm = D.nrparm;
+  d.ilocu = D.ilocu; d.ilocv = D.ilocv; d.ilocw = D.ilocw;
+  d.ilocb = D.ilocb; d.iloca1 = D.iloca1; d.iloca2 = D.iloca2;
+  d.nchan = D.nchan; d.incf = D.incf; d.incif = D.incif;
+  d.nchunk = (D.nchan + beam::kCPT - 1) / beam::kCPT;
+  const long long nrowblk = (nrow + beam::kRows - 1) / beam::kRows;
+  if (nrowblk > 0x7fffffffLL) return fail(kInvalid, "ObitHIPBeamDFTCalc: too many rows");
+  d.nrowblk = (int)nrowblk;
+  d.nitems = nrowblk * D.nif * d.nchunk;
+  d.ncomp = D.ncomp; d.ntype = D.ntype; d.nant = D.nant; d.njchan = D.njchan;
+  d.npair = D.ntype * D.ntype;
+  d.fIF = h->d_freq; d.dfIF = h->d_freq + D.nif;
+  d.cx = h->d_pos; d.cy = h->d_pos + D.ncomp; d.cz = h->d_pos + 2 * (size_t)D.ncomp;
+  d.camp = h->d_amp; d.cg = h->d_g;
+  d.antType = h->d_antType; d.jmap = h->d_jmap;
+  d.coh = h->d_coh;
+  d.op = D.opType;
+  for (int i = 0; i < 4; ++i) d.sf[i] = D.stokFact[i];
+  long long grid = (long long)std::max(h->ncu, 1) * 2;
+  if (grid > d.nitems) grid = d.nitems;
+  void *args[] = {&d};
+  const void *fn;
+  if (D.ncorr == 2) fn = h->gauss ? (const void *)beam::k_beam_dft<2, true> : (const void *)beam::k_beam_dft<2, false>;
+  else fn = h->gauss ? (const void *)beam::k_beam_dft<4, true> : (const void *)beam::k_beam_dft<4, false>;
+  OBIT_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(beam::kRows), args, 0, h->stream));
+  return kOK;
+}
+
+int ObitHIPBeamDFTDestroy(ObitHIPBeam *h) {
+  if (h && h->stream) (void)hipStreamSynchronize(h->stream);
+  else if (h) (void)hipDeviceSynchronize();
+  release(h);
+  return obit::kOK;
+}
+
+}  // extern "C"

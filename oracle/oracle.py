@@ -383,3 +383,125 @@ def stokes_i(buffer, nrparm, nif, nchan, nstok, incs, incf, incif, kstoke0, form
                 out[:, o + 1] = np.where(live, vi, zero)
                 out[:, o + 2] = wt
     return out
+
+
+# ------------------------------------------------------- Jones-corrected DFT
+
+def beam_ants(buffer, ilocb=3, iloca1=-1, iloca2=-1):
+    """0-rel antenna numbers of each row: BASELINE = 256 a1 + a2 (+ subarray
+    fraction), truncated as GetAnts does (reference
+    src/ObitCUDABeamModel.cu:82-98), or ANTENNA1/ANTENNA2 rounded."""
+    if ilocb >= 0:
+        cb = buffer[:, ilocb].astype(np.float32).astype(np.int64)   # float -> int truncation
+        a1 = (cb / 256.0 + 0.001).astype(np.int64)
+        a2 = ((cb - a1 * 256) + 0.001).astype(np.int64)
+    else:
+        a1 = (buffer[:, iloca1] + np.float32(0.5)).astype(np.int64)
+        a2 = (buffer[:, iloca2] + np.float32(0.5)).astype(np.int64)
+    return a1 - 1, a2 - 1
+
+
+def beam_coherency(stok_type, par_ang):
+    """2x2 coherency of a unit model in Stokes I, Q, U or V for linear feeds
+    at parallactic angle par_ang, as Obit's CPU model sets it
+    (src/ObitSkyModelVMBeamMF.c:2722-2747, TMS2 eq. 4.41 order XX, XY, YX, YY)."""
+    c, s = np.cos(2.0 * par_ang), np.sin(2.0 * par_ang)
+    if stok_type == 2:
+        return np.array([[c, -s], [-s, -c]], np.complex128)
+    if stok_type == 3:
+        return np.array([[s, c], [c, -s]], np.complex128)
+    if stok_type == 4:
+        return np.array([[0, 1j], [-1j, 0]], np.complex128)
+    return np.eye(2, dtype=np.complex128)
+
+
+def beam_products(jones, ant_types_pairs, stok_type, par_ang):
+    """J1 P J2^H per (pair, component, beam channel): beam1 * model *
+    beam2^H (src/ObitSkyModelVMBeamMF.c:2752-2755).  jones: complex
+    [ntype, ncomp, njchan, 2, 2]."""
+    P = beam_coherency(stok_type, par_ang)
+    out = {}
+    for (t1, t2) in ant_types_pairs:
+        J1, J2 = jones[t1], jones[t2]
+        out[(t1, t2)] = J1 @ P @ np.conj(np.swapaxes(J2, -1, -2))
+    return out
+
+
+def beam_model_fp64(buffer, comps, freq_scale, ant_type, vis_jchan, jones, stok_type=1, par_ang=0.0,
+                    ilocu=0, ilocv=1, ilocw=2, ilocb=3, iloca1=-1, iloca2=-1, stok_fact=(1, 1, 1, 1)):
+    """FP64 model [nrow, nif*nchan, 2, 2] (XX XY / YX YY) of the
+    Jones-corrected DFT, and the mask of rows whose antennas are in the
+    type table.  Phase u x + v y + w z scaled by nu/nu0 (radians, x in
+    radians per wavelength); a Gaussian multiplies the amplitude by
+    exp((nu/nu0)^2 (g0 u^2 + g1 v^2 + g2 uv)) (reference
+    src/ObitCUDABeamModel.cu:197-203, 761-767).  freq_scale: nu/nu0 per
+    (IF, channel), IF-major."""
+    buf = np.asarray(buffer, np.float32)
+    comps = np.asarray(comps, np.float32).astype(np.float64)
+    a1, a2 = beam_ants(buf, ilocb, iloca1, iloca2)
+    nant = len(ant_type)
+    live = (a1 >= 0) & (a1 < nant) & (a2 >= 0) & (a2 < nant)
+    at = np.asarray(ant_type)
+    t1 = np.where(live, at[np.clip(a1, 0, nant - 1)], 0)
+    t2 = np.where(live, at[np.clip(a2, 0, nant - 1)], 0)
+    pairs = sorted(set(zip(t1.tolist(), t2.tolist())))
+    prods = beam_products(jones, pairs, stok_type, par_ang)
+    u = buf[:, ilocu].astype(np.float64)
+    v = buf[:, ilocv].astype(np.float64)
+    w = buf[:, ilocw].astype(np.float64)
+    psi = np.outer(u, comps[:, 1]) + np.outer(v, comps[:, 2]) + np.outer(w, comps[:, 3])
+    gauss = comps.shape[1] >= 7 and np.any(comps[:, 4:7] != 0)
+    G = (np.outer(u * u, comps[:, 4]) + np.outer(v * v, comps[:, 5]) + np.outer(u * v, comps[:, 6])) if gauss else None
+    nrow, nf = buf.shape[0], len(freq_scale)
+    out = np.zeros((nrow, nf, 2, 2), np.complex128)
+    rows_of = {p: np.nonzero((t1 == p[0]) & (t2 == p[1]))[0] for p in pairs}
+    for f in range(nf):
+        fs = float(freq_scale[f])
+        amp = comps[:, 0][None, :] * (np.exp(fs * fs * G) if gauss else 1.0)
+        V = amp * np.exp(1j * fs * psi)                       # [nrow, ncomp]
+        jc = int(vis_jchan[f])
+        for p, r in rows_of.items():
+            if len(r):
+                out[r, f] = np.einsum("rk,kab->rab", V[r], prods[p][:, jc])
+    sf = np.asarray(stok_fact, np.float64).reshape(2, 2)
+    return out * sf[None, None], live
+
+
+def beam_apply(buffer, model, live, nrparm, nif, nchan, incf, incif, ncorr, op="sub"):
+    """Apply a [nrow, nif*nchan, 2, 2] model to a copy of the rows as
+    Obit's CPU beam DFT does (src/ObitSkyModelVMBeamMF.c:3711-3800), in
+    FP32: channels with every correlation flagged are skipped unless
+    replacing; each correlation changes where its weight is positive (or
+    always when replacing); divide acts on XX and YY only; cross hands only
+    with 4 correlations."""
+    buf = np.array(buffer, np.float32, copy=True)
+    m = model.astype(np.complex64)
+    repl = op == "repl"
+    hands = [(0, 0, 0), (1, 1, 3)] + ([(0, 1, 6), (1, 0, 9)] if ncorr == 4 and op != "div" else [])
+    for i in range(nif):
+        for c in range(nchan):
+            f = i * nchan + c
+            o = nrparm + i * incif + c * incf
+            wts = [buf[:, o + 2], buf[:, o + 5]] + ([buf[:, o + 8], buf[:, o + 11]] if ncorr == 4 else [])
+            bad = np.all(np.stack(wts) <= 0, axis=0)
+            act = live & (~bad | repl)
+            for (a, b, off) in hands:
+                q = o + off
+                mr = np.real(m[:, f, a, b]).astype(np.float32)
+                mi = np.imag(m[:, f, a, b]).astype(np.float32)
+                sel = act & ((buf[:, q + 2] > 0) | repl)
+                if op == "div":
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        wm = mr * mr + mi * mi
+                        dr, di = mr / wm, mi / wm
+                        re, im = buf[:, q].copy(), buf[:, q + 1].copy()
+                        nr = dr * re + di * im
+                        ni = dr * im - di * re
+                        nw = buf[:, q + 2] * np.sqrt(wm)
+                    buf[sel, q], buf[sel, q + 1], buf[sel, q + 2] = nr[sel], ni[sel], nw[sel]
+                elif repl:
+                    buf[sel, q], buf[sel, q + 1] = mr[sel], mi[sel]
+                else:
+                    buf[sel, q] -= mr[sel]
+                    buf[sel, q + 1] -= mi[sel]
+    return buf

@@ -1,0 +1,147 @@
+"""Jones-corrected DFT on the GPU (ObitHIPBeamDFT*) against the FP64
+oracle (oracle.beam_model_fp64 + beam_apply).
+
+Cases cover 2 and 4 correlations, point and Gaussian tables, the four
+Stokes models, subtract / replace / divide, beam-channel maps that keep an
+8-channel chunk in one beam channel (LDS path) and maps that do not
+(per-channel products), more antenna types than the LDS path stages, rows
+whose antennas fall outside the type table, and flagged samples.
+Tolerance: the model is within 1e-5 of its peak amplitude (the split DFT's
+bound); entries the application rules leave alone are bit-identical."""
+import numpy as np
+import pytest
+
+from obit_amd import synth
+from obit_amd.beam import BeamDFT
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+NANT = 27
+
+
+def _data(ncorr, rows, nif=2, nchan=40, seed=0):
+    """Workload 1's uvw and baselines, ncorr correlations per channel of
+    N(0,1) data with unit weights, 8 % of samples flagged, 2 % of rows with
+    an antenna outside the 27-entry type table."""
+    rng = np.random.default_rng(seed)
+    base, _, _ = synth.workload(1, rows=rows)
+    nrparm = base.nrparm
+    lrec = nrparm + nif * nchan * 3 * ncorr
+    buf = np.zeros((rows, lrec), np.float32)
+    buf[:, :nrparm] = base.buffer[:, :nrparm]
+    vis = buf[:, nrparm:].reshape(rows, nif, nchan, ncorr, 3)
+    vis[..., 0] = rng.normal(size=vis.shape[:-1])
+    vis[..., 1] = rng.normal(size=vis.shape[:-1])
+    vis[..., 2] = np.where(rng.uniform(size=vis.shape[:-1]) < 0.08, -1.0, 1.0)
+    vis[rng.uniform(size=(rows, nif, nchan)) < 0.02, :, 2] = 0.0     # whole channels flagged
+    bad = rng.uniform(size=rows) < 0.02
+    cb = buf[:, base.ilocb].astype(np.int64)
+    buf[bad, base.ilocb] = ((cb[bad] // 256) * 256 + 31).astype(np.float32)   # antenna 31
+    f0 = base.freq
+    freqArr = (f0 * (1.0 + 0.02 * np.arange(nif))[:, None] + 250e3 * np.arange(nchan)[None, :]).ravel()
+    uv = synth.UVData(buffer=buf, nchan=nchan, nif=nif, nstok=ncorr, freq=f0, freqArr=freqArr,
+                      crval_stokes=-5.0, nrparm=nrparm)
+    return uv
+
+
+def _comps(gauss, ncomp=48, seed=1):
+    rng = np.random.default_rng(seed)
+    _, sky, _ = synth.workload(1, rows=8)
+    c = sky.comps[:ncomp].astype(np.float64).copy()
+    c[0, 0] = 1.0                                  # a dominant component (division)
+    if gauss:
+        g = np.zeros((ncomp, 3))
+        isg = rng.uniform(size=ncomp) < 0.4
+        maj = rng.uniform(2.0, 30.0, ncomp) / 3600.0
+        gp = np.stack(synth.gauss_coeffs(maj, maj * rng.uniform(0.3, 1.0, ncomp),
+                                         rng.uniform(0, 180, ncomp)), axis=-1)
+        g[isg] = gp[isg]
+        c = np.concatenate([c, g], axis=1)
+    return c.astype(np.float32)
+
+
+def _jones(ntype, ncomp, njchan, seed=2):
+    rng = np.random.default_rng(seed)
+    j = np.zeros((ntype, ncomp, njchan, 2, 2), np.complex128)
+    j[..., 0, 0] = 1.0
+    j[..., 1, 1] = 1.0
+    j += 0.15 * (rng.normal(size=j.shape) + 1j * rng.normal(size=j.shape))
+    return j.astype(np.complex64).astype(np.complex128)      # the device holds FP32
+
+
+CASES = [
+    # ncorr, gauss, stokes, op, uniform beam channels, antenna types
+    (2, False, 1, "sub", True, 2),
+    (2, True, 2, "repl", False, 2),
+    (4, False, 3, "sub", True, 3),
+    (4, True, 4, "repl", False, 2),
+    (2, False, 1, "div", True, 1),
+    (4, True, 2, "div", False, 2),
+    (4, False, 4, "sub", True, 5),      # 25 type pairs: per-channel products
+    (2, True, 3, "sub", False, 1),
+]
+
+
+@pytest.mark.parametrize("ncorr,gauss,stok,op,uniform,ntype", CASES)
+def test_beam_dft_matches_oracle(ncorr, gauss, stok, op, uniform, ntype):
+    uv = _data(ncorr, rows=1500, seed=ncorr + 10 * stok)
+    comps = _comps(gauss)
+    rng = np.random.default_rng(3)
+    ant_type = rng.integers(0, ntype, NANT)
+    nf = uv.nif * uv.nchan
+    ch = np.arange(nf) % uv.nchan
+    jmap = (np.arange(nf) // uv.nchan) * 5 + (ch // 8 if uniform else ch // 6)
+    njchan = int(jmap.max()) + 1
+    jones = _jones(ntype, len(comps), njchan)
+    chi = 0.41
+    sf = (1.0, 0.9, 0.8, 1.1)
+    with BeamDFT(uv, comps, ant_type, jmap, njchan, ntype=ntype, stok_type=stok, op=op,
+                 stok_fact=sf) as bm:
+        bm.set_jones(jones)
+        got = bm.calc(uv.buffer, par_ang=chi)
+    model, live = orc.beam_model_fp64(uv.buffer, comps, uv.freqArr / uv.freq, ant_type, jmap, jones, stok,
+                                      chi, ilocb=uv.ilocb, stok_fact=sf)
+    want = orc.beam_apply(uv.buffer, model, live, uv.nrparm, uv.nif, uv.nchan, uv.incf, uv.incif, ncorr, op)
+    assert (~live).sum() > 0 and live.sum() > 0
+    # the entries the rules select: a NaN model marks them
+    moved = np.isnan(orc.beam_apply(uv.buffer, np.full_like(model, complex(np.nan, np.nan)), live, uv.nrparm, uv.nif,
+                                    uv.nchan, uv.incf, uv.incif, ncorr, op))
+    assert np.array_equal(got[~moved], uv.buffer[~moved]), "entries the model must not touch changed"
+    peak = np.abs(model[live]).max()
+    err = np.abs(got[moved].astype(np.float64) - want[moved]).max()
+    if op == "div":
+        scale = np.abs(want[moved]).max()
+        assert err < 2e-5 * scale, (err, scale)
+    else:
+        assert err < 1e-5 * peak, (err, peak)
+
+
+def test_beam_unit_jones_matches_split_dft():
+    """Unit Jones, Stokes I, 2 correlations: the subtracted model is the
+    plain DFT's (the oracle's FP64 model of the same table)."""
+    uv = _data(2, rows=1200, nif=1, nchan=64, seed=5)
+    comps = _comps(False)
+    with BeamDFT(uv, comps, np.zeros(NANT, int), np.zeros(uv.nchan, int), 1, stok_type=1, op="sub") as bm:
+        got = bm.calc(uv.buffer)
+    sky = synth.SkyModel(comps=comps, modType=0)
+    plain = orc.dft_model_fp64(uv, sky)                          # [nrow, nif, nchan]
+    cb = uv.buffer[:, uv.ilocb].astype(np.int64)
+    live = (cb // 256 <= NANT) & (cb % 256 <= NANT)
+    d0 = uv.data_view()
+    d1 = got[:, uv.nrparm:].reshape(d0.shape)
+    for h in range(2):
+        wt = d0[..., h, 2] > 0
+        sel = wt & live[:, None, None]
+        mod = (d0[..., h, 0] - d1[..., h, 0]) + 1j * (d0[..., h, 1] - d1[..., h, 1])
+        assert np.abs(mod[sel] - plain[sel]).max() < 1e-5 * np.abs(plain).max()
+        assert np.all(mod[~sel] == 0)
+
+
+def test_beam_refuses_bad_tables():
+    uv = _data(2, rows=64, nif=1, nchan=8)
+    comps = _comps(False, ncomp=4)
+    with pytest.raises(Exception, match="type"):
+        BeamDFT(uv, comps, np.full(NANT, 3), np.zeros(8, int), 1, ntype=2)
+    with pytest.raises(Exception, match="beam channel"):
+        BeamDFT(uv, comps, np.zeros(NANT, int), np.full(8, 4), 2)
