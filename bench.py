@@ -248,6 +248,77 @@ def bench_fft2image(lib, gg, facets):
             "min_traffic_GBps": mib * 2**20 / (ms * 1e-3) / 1e9}
 
 
+def bench_beam(lib, uv, sky, reps=2, check_rows=16, check=True):
+    """The Jones-corrected DFT (ObitHIPBeamDFT*) on workload 2 at full size:
+    every resident row, XX/YY, the 10,000-point table, one antenna type
+    (a homogeneous array), 8 beam channels of 32 channels each, seeded
+    near-unit Jones matrices, Stokes I, subtract; HIP events on the default
+    stream.  With check (the CPU leg), a seeded row sample of the first
+    launch is compared with the oracle's FP64 model (checker only, outside
+    the timing)."""
+    from obit_amd import _lib
+    from obit_amd.beam import BeamDFT
+    cb = uv.buffer[:, uv.ilocb].astype(np.int64)
+    nant = int(max((cb // 256).max(), (cb % 256).max()))
+    nf = uv.nif * uv.nchan
+    njchan = 8
+    jmap = (np.arange(nf) * njchan) // nf
+    comps = np.ascontiguousarray(sky.comps[:, :4], np.float32)
+    rng = np.random.default_rng(11)
+    jones = np.zeros((1, comps.shape[0], njchan, 2, 2), np.complex64)
+    jones[..., 0, 0] = jones[..., 1, 1] = 1.0
+    jones += (0.1 * (rng.normal(size=jones.shape) + 1j * rng.normal(size=jones.shape))).astype(np.complex64)
+    evals = uv.nrow * nf * comps.shape[0]
+    rows = np.sort(np.random.default_rng(12).choice(uv.nrow, check_rows, replace=False))
+    d = C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(d), uv.buffer.nbytes), "malloc vis")
+    try:
+        _lib.check(lib.ObitHIPMemcpyH2D(d, _lib.ptr(uv.buffer), uv.buffer.nbytes), "upload vis")
+        with BeamDFT(uv, comps, np.zeros(nant, np.int32), jmap, njchan, ntype=1, stok_type=1, op="sub") as bm:
+            bm.set_jones(jones)
+            bm.calc_device(d.value, uv.nrow, 0.3)            # warm; its rows are checked
+            lib.ObitHIPDeviceSynchronize()
+            got = np.empty((rows.size, uv.lrec), np.float32)
+            for i, r in enumerate(rows):
+                _lib.check(lib.ObitHIPMemcpyD2H(got[i].ctypes.data, d.value + int(r) * uv.lrec * 4, uv.lrec * 4),
+                           "D2H")
+            e0, e1 = _events(lib)
+            lib.ObitCUDAEventRecord(e0, None)
+            for _ in range(reps):
+                bm.calc_device(d.value, uv.nrow, 0.3)
+            lib.ObitCUDAEventRecord(e1, None)
+            lib.ObitHIPDeviceSynchronize()
+            ms = lib.ObitHIPEventElapsedMs(e0, e1) / reps
+    finally:
+        lib.ObitHIPFree(d)
+    flop = 24.0   # phasor step 6, amplitude 2, two complex multiply-adds 16
+    res = {"workload": "workload 2 rows (907,200 x 256 ch, XX/YY) x 10,000 points, 1 antenna type, "
+                       "8 beam channels, Stokes I, subtract",
+           "rows": uv.nrow, "channels": nf, "components": int(comps.shape[0]), "beam_channels": njchan,
+           "ms": ms, "evals_per_s": evals / (ms * 1e-3),
+           "roofline": {"bound": "valu", "achieved": evals * flop / (ms * 1e-3) / 1e12, "peak": 157.3,
+                        "unit": "TFLOP/s", "frac": evals * flop / (ms * 1e-3) / 1e12 / 157.3,
+                        "flop_per_eval": flop}}
+    if check:
+        from oracle import oracle as orc
+        sub = uv.buffer[rows]
+        m, live = orc.beam_model_fp64(sub, comps, uv.freqArr / uv.freq, np.zeros(nant, int), jmap,
+                                      jones.astype(np.complex128), 1, 0.3, ilocb=uv.ilocb)
+        d0 = sub[:, uv.nrparm:].reshape(rows.size, nf, uv.nstok, 3)
+        d1 = got[:, uv.nrparm:].reshape(d0.shape)
+        sel = d0[..., 2] > 0
+        diff = 0.0
+        for h in range(2):
+            mod = (d0[:, :, h, 0] - d1[:, :, h, 0]).astype(np.float64) + 1j * (d0[:, :, h, 1] - d1[:, :, h, 1])
+            diff = max(diff, float(np.abs(mod - m[:, :, h, h])[sel[..., h]].max()))
+        res["err_max_vs_fp64_rel_peak"] = diff / float(np.abs(m).max())
+        res["check_rows"] = check_rows
+    log(f"beam DFT: {evals:.3e} evaluations in {ms:.1f} ms = {res['evals_per_s']:.3e}/s "
+        f"({res['roofline']['frac'] * 100:.1f} % of FP32 at {flop:.0f} FLOP each)"
+        + (f"; max|dV|/peak {res['err_max_vs_fp64_rel_peak']:.2e}" if check else ""))
+    return res
+
+
 def bench_stokes(lib, uv, reps=5, check_rows=2000, check=True):
     """Stokes I records of workload 2's resident RR/LL rows
     (ObitHIPStokesI, the gridder's input), timed with HIP events on the
@@ -592,6 +663,7 @@ def main():
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid-only", action="store_true", help="time the gridding workload alone")
+    ap.add_argument("--beam-only", action="store_true", help="time the Jones-corrected DFT alone")
     ap.add_argument("--workload5", action="store_true",
                     help="run workload 5 at full size (89 GB resident; about two minutes) and exit")
     args = ap.parse_args()
@@ -611,6 +683,11 @@ def main():
     if args.workload5:
         print(json.dumps({"workload5": bench_workload5(lib)}), flush=True)
         return
+    if args.beam_only:
+        from obit_amd import synth
+        uv, sky, _ = synth.workload(2)
+        print(json.dumps({"beam_dft": bench_beam(lib, uv, sky, check=not args.no_cpu)}), flush=True)
+        return
     if args.grid_only:
         grid, _ = bench_grid(lib, args.grid_steps)
         print(json.dumps({"gridding": grid}), flush=True)
@@ -622,6 +699,7 @@ def main():
     pcie = None if args.no_accuracy else bench_pcie(lib, uv, sky, vt)
     stokes = None if args.no_grid else bench_stokes(lib, uv, check=not args.no_cpu)
     batched = None if args.no_grid else bench_batched(lib)
+    beam = None if args.no_grid else bench_beam(lib, uv, sky, check=not args.no_cpu)
     grid, gwork = (None, None) if args.no_grid else bench_grid(lib, args.grid_steps)
     cpu = None if args.no_cpu else cpu_baseline(uv, sky, args.cpu_seconds)
     if grid is not None and not args.no_cpu:
@@ -676,6 +754,7 @@ def main():
         "gridding": grid,
         "stokes_i": stokes,
         "batched_multifacet": batched,
+        "beam_dft": beam,
     }
     print(json.dumps(line), flush=True)
 

@@ -13,7 +13,7 @@
 // phasor into each correlation:
 //   k_beam_dft  S_e(row, chan) = sum_k a_k exp(2 pi i psi_k nu/nu0) C_e
 // with the phasor walk of the split DFT (FP64 phase and range reduction,
-// FP32 sincos at the centre of an 8-channel chunk, the step as three
+// FP32 sincos at the centre of the thread's channel chunk, the step as three
 // shears) and FP32 tile sums folded into FP64.
 //
 // Work decomposition: a workgroup is 256 consecutive rows x one (IF,
@@ -34,7 +34,10 @@ namespace obit {
 namespace beam {
 
 constexpr int kRows = 256;     // rows (threads) per workgroup
-constexpr int kCPT = 8;        // channels per thread
+// Channels per thread.  16 with 2 correlations needs 310 VGPRs (one wave
+// per SIMD) and ran 1.54x slower than 8 (two waves per SIMD).
+constexpr int kCPT2 = 8;       // 2 correlations
+constexpr int kCPT4 = 8;       // 4 correlations
 constexpr int kTile = 64;      // components per LDS tile
 constexpr int kMaxPairs = 16;  // type pairs staged in LDS (4 antenna types)
 constexpr int kMaxTypes = 16;
@@ -145,7 +148,7 @@ __device__ __forceinline__ int prod_index(int e) {
   return NC == 2 ? 3 * e : e;
 }
 
-template <int NC, bool GAUSS>
+template <int NC, int kCPT, bool GAUSS>
 __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
   __shared__ double s_x[kTile], s_y[kTile], s_z[kTile];
   __shared__ float s_a[kTile];
@@ -521,7 +524,8 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   d.ilocu = D.ilocu; d.ilocv = D.ilocv; d.ilocw = D.ilocw;
   d.ilocb = D.ilocb; d.iloca1 = D.iloca1; d.iloca2 = D.iloca2;
   d.nchan = D.nchan; d.incf = D.incf; d.incif = D.incif;
-  d.nchunk = (D.nchan + beam::kCPT - 1) / beam::kCPT;
+  const int cpt = D.ncorr == 2 ? beam::kCPT2 : beam::kCPT4;
+  d.nchunk = (D.nchan + cpt - 1) / cpt;
   const long long nrowblk = (nrow + beam::kRows - 1) / beam::kRows;
   if (nrowblk > 0x7fffffffLL) return fail(kInvalid, "ObitHIPBeamDFTCalc: too many rows");
   d.nrowblk = (int)nrowblk;
@@ -539,8 +543,10 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   if (grid > d.nitems) grid = d.nitems;
   void *args[] = {&d};
   const void *fn;
-  if (D.ncorr == 2) fn = h->gauss ? (const void *)beam::k_beam_dft<2, true> : (const void *)beam::k_beam_dft<2, false>;
-  else fn = h->gauss ? (const void *)beam::k_beam_dft<4, true> : (const void *)beam::k_beam_dft<4, false>;
+  if (D.ncorr == 2)
+    fn = h->gauss ? (const void *)beam::k_beam_dft<2, beam::kCPT2, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, false>;
+  else
+    fn = h->gauss ? (const void *)beam::k_beam_dft<4, beam::kCPT4, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, false>;
   OBIT_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(beam::kRows), args, 0, h->stream));
   return kOK;
 }
