@@ -6,9 +6,9 @@
  * (beamPointKernel :139-310, beamGaussKernel :687-858) behind entry points
  * that follow ObitCUDABeamModelDFTInit / SetMod / Calc / Shutdown
  * (:1368-1437; include/ObitCUDABeamModel.h), with the Jones table that
- * GetJonesKernel / UnitJonesKernel (:1227-1340) leave in d_Jones handed in
- * by the caller (or the unit matrix).  Beam-image interpolation is not part
- * of this library.
+ * GetJonesKernel / UnitJonesKernel (:1227-1340) leave in d_Jones either
+ * handed in by the caller, set to the unit matrix, or formed on the device
+ * from beam images (ObitHIPBeamDFTSetBeams / ObitHIPBeamDFTJonesFromBeams).
  *
  * For every (row, IF, channel) the model is
  *     S = stokFact (.) sum_k a_k(nu) exp(2 pi i psi_k nu/nu0) J1_k P J2_k^H
@@ -95,6 +95,41 @@ int ObitHIPBeamDFTSetJones(ObitHIPBeam *h, const float *jones);
  * parAng (radians) (ObitCUDABeamModelDFTCalc).  Asynchronous on the
  * handle's stream. */
 int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAng);
+/* Copy the current Jones table to the host ([ntype][ncomp][njchan][8]). */
+int ObitHIPBeamDFTGetJones(ObitHIPBeam *h, float *jones);
+
+/* Beam images of one antenna type (the reference's eight CUDABeamInterp
+ * objects per type, src/ObitGPUBeamModel.c:689-725): axes nx x ny x nchan
+ * (nchan = njchan), the geometry of its image descriptor, and the
+ * Lagrange half width of the interpolation (the reference uses 2). */
+typedef struct {
+  int nx, ny, nchan;
+  int proj;          /* 0 linear, 1 -SIN, 2 -TAN, 3 -ARC (others refused) */
+  int hwidth;        /* 1..4 */
+  double crval[2];   /* reference position (deg) */
+  float crpix[2];    /* reference pixel (1-rel) */
+  float cdelt[2];    /* increments (deg) */
+} ObitHIPBeamImageDesc;
+
+/* Load the beam images: images [ntype][8][nchan][ny][nx] floats, per type
+ * in the reference's order P1 re, P1 im, P2 re, P2 im, X1 re, X1 im,
+ * X2 re, X2 im (RX, RXIm, LY, LYIm, RL, RLIm, LR, LRIm); beamFreq [njchan]
+ * the beam planes' frequencies (Hz); refSizeFact the primary-beam factor
+ * of the reference's InvBeam (antenna size x fudge x deg->rad / c);
+ * fblank the blanking value of the images. */
+int ObitHIPBeamDFTSetBeams(ObitHIPBeam *h, const ObitHIPBeamImageDesc *desc, const float *images,
+                           const float *beamFreq, float refSizeFact, float fblank);
+/* Form the Jones table from the beam images for the components at pos
+ * [ncomp][2] (RA and Dec offsets, deg, as Obit's beam-model table holds
+ * them) and parallactic angle parAng (radians), as GetJonesKernel does
+ * (ObitCUDABeamModel.cu:1227-1308): the position (-RA, Dec) is placed in
+ * each type's beam image rotated by parAng; each element is the
+ * interpolated image value times 1/sqrt of the reference beam gain at the
+ * offset and beam frequency; a position off the image, or a reference
+ * gain or P1 beyond the reference's limits, gives the unit matrix.
+ * Asynchronous on the handle's stream. */
+int ObitHIPBeamDFTJonesFromBeams(ObitHIPBeam *h, const float *pos, float parAng);
+
 /* Release the handle (ObitCUDABeamModelDFTShutdown). */
 int ObitHIPBeamDFTDestroy(ObitHIPBeam *h);
 

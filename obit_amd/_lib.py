@@ -42,6 +42,13 @@ class BeamDesc(C.Structure):
         "njchan", "stokType", "opType")] + [("stokFact", C.c_float * 4)]
 
 
+class BeamImageDesc(C.Structure):
+    """ObitHIPBeamImageDesc (include/ObitHIPBeam.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nchan", C.c_int), ("proj", C.c_int),
+                ("hwidth", C.c_int), ("crval", C.c_double * 2), ("crpix", C.c_float * 2),
+                ("cdelt", C.c_float * 2)]
+
+
 class HostSkyModel(C.Structure):
     _fields_ = [
         ("modType", C.c_int), ("nSpecTerm", C.c_int),
@@ -161,6 +168,9 @@ _PROTOS = {
     "ObitHIPBeamDFTSetMod": (_I, [_P, _P]),
     "ObitHIPBeamDFTSetJones": (_I, [_P, _P]),
     "ObitHIPBeamDFTCalc": (_I, [_P, _P, _LL, _F]),
+    "ObitHIPBeamDFTGetJones": (_I, [_P, _P]),
+    "ObitHIPBeamDFTSetBeams": (_I, [_P, _P, _P, _P, _F, _F]),
+    "ObitHIPBeamDFTJonesFromBeams": (_I, [_P, _P, _F]),
     "ObitHIPBeamDFTDestroy": (_I, [_P]),
     # host classes (csrc/host.h)
     "ObitHostDFTCreate": (_P, []),

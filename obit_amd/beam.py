@@ -68,6 +68,35 @@ class BeamDFT:
         flat = np.ascontiguousarray(j.reshape(j.shape[:3] + (4,)).view(np.float32))
         _lib.check(self.lib.ObitHIPBeamDFTSetJones(self.h, flat.ctypes.data), "ObitHIPBeamDFTSetJones")
 
+    def get_jones(self):
+        """The device's Jones table as complex [ntype, ncomp, njchan, 2, 2]."""
+        d = self.desc
+        out = np.empty((d.ntype, max(d.ncomp, 1), d.njchan, 8), np.float32)
+        _lib.check(self.lib.ObitHIPBeamDFTGetJones(self.h, out.ctypes.data), "ObitHIPBeamDFTGetJones")
+        return out.view(np.complex64).reshape(d.ntype, max(d.ncomp, 1), d.njchan, 2, 2)
+
+    def set_beams(self, descs, images, beam_freq, ref_size_fact, fblank):
+        """descs: per type dicts (nx, ny, proj, hwidth, crval, crpix, cdelt);
+        images float32 [ntype, 8, njchan, ny, nx] (or a list of per-type
+        arrays [8, njchan, ny, nx]); beam_freq [njchan] Hz."""
+        arr = (_lib.BeamImageDesc * len(descs))()
+        for a, dd in zip(arr, descs):
+            a.nx, a.ny, a.nchan = dd["nx"], dd["ny"], self.desc.njchan
+            a.proj, a.hwidth = dd.get("proj", 0), dd.get("hwidth", 2)
+            for i in range(2):
+                a.crval[i], a.crpix[i], a.cdelt[i] = dd["crval"][i], dd["crpix"][i], dd["cdelt"][i]
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).ravel() for x in images]))
+        bf = np.ascontiguousarray(beam_freq, np.float32)
+        self._beams = (arr, flat, bf)
+        _lib.check(self.lib.ObitHIPBeamDFTSetBeams(self.h, arr, flat.ctypes.data, bf.ctypes.data,
+                                                   float(ref_size_fact), float(fblank)), "ObitHIPBeamDFTSetBeams")
+
+    def jones_from_beams(self, pos, par_ang=0.0):
+        """pos: float32 [ncomp, 2] RA and Dec offsets (deg)."""
+        p = np.ascontiguousarray(pos, np.float32)
+        _lib.check(self.lib.ObitHIPBeamDFTJonesFromBeams(self.h, p.ctypes.data, float(par_ang)),
+                   "ObitHIPBeamDFTJonesFromBeams")
+
     def calc_device(self, d_vis, nrow, par_ang=0.0):
         _lib.check(self.lib.ObitHIPBeamDFTCalc(self.h, C.c_void_p(d_vis), nrow, float(par_ang)),
                    "ObitHIPBeamDFTCalc")

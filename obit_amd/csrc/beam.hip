@@ -370,6 +370,161 @@ __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
   }
 }
 
+// ---- Jones matrices from beam images (the reference's GetJonesKernel,
+// ObitCUDABeamModel.cu:1227-1308, with CUDABeamInterpPixel and
+// SetConvKernal, include/CUDABeamInterp.h:81-145, 233-258, and
+// CUDAPositionXYpix(LM), include/CUDASkyGeom.h:90-312)
+
+struct BeamImg {
+  int nx, ny, proj, hwidth;
+  double crval[2];
+  float crpix[2], cdelt[2];
+  float denom[10];      // reciprocals of the Lagrange denominators
+  long long off;        // first float of this type's 8 images
+};
+
+// Pixel (1-rel) of a position (deg) in a beam image rotated by rot
+// (radians); nonzero when the projection cannot place it.
+__device__ int beam_pixel(const BeamImg &b, double x, double y, float rot, float *px, float *py) {
+  const double dt = x - b.crval[0];
+  if (dt > 180.0) x -= 360.0;
+  if (dt < -180.0) x += 360.0;
+  double d0 = x - b.crval[0], d1 = y - b.crval[1];
+  if (b.cdelt[0] == 0.0f || b.cdelt[1] == 0.0f) return 2;
+  if (b.proj != 0) {
+    const double d2r = 1.745329252e-2;   // the reference's factor
+    const double ra0 = b.crval[0] * d2r, dec0 = b.crval[1] * d2r, ra = x * d2r, dec = y * d2r;
+    const double coss = cos(dec), sins = sin(dec), cda = cos(ra - ra0);
+    double l = sin(ra - ra0) * coss;
+    const double sint = sins * sin(dec0) + coss * cos(dec0) * cda;
+    const double mm = sins * cos(dec0) - coss * sin(dec0) * cda;
+    double m;
+    if (b.proj == 1) {            // -SIN
+      if (sint < 0.0) return 1;
+      m = mm;
+    } else if (b.proj == 2) {     // -TAN
+      if (sint <= 0.0) return 1;
+      l = l / sint;
+      m = mm / sint;
+    } else {                      // -ARC
+      double c = fmin(1.0, fmax(-1.0, sint));
+      c = acos(c);
+      c = (c != 0.0) ? c / sin(c) : 1.0;
+      l = l * c;
+      m = mm * c;
+    }
+    d0 = l / d2r;
+    d1 = m / d2r;
+  }
+  float sr, cr;
+  sincosf(rot, &sr, &cr);
+  const double dz = d0 * cr + d1 * sr;
+  d1 = d1 * cr - d0 * sr;
+  d0 = dz;
+  *px = (float)(d0 / b.cdelt[0] + b.crpix[0]);
+  *py = (float)(d1 / b.cdelt[1] + b.crpix[1]);
+  return 0;
+}
+
+// Lagrange weights for 1-rel pixel p on an axis of n pixels.
+__device__ __forceinline__ void lagrange(float p, int n, int hw, const float *denom, int *start, float *k) {
+  const int iwid = 2 * hw + 1;
+  int cen = (int)(p + 0.5f) - hw;
+  cen = max(1, min(cen, n - iwid + 1)) - 1;
+  *start = cen;
+  const float xx = p - (float)cen;
+  for (int j = 0; j < iwid; ++j) {
+    float prod = denom[j];
+    for (int i = 0; i < iwid; ++i)
+      if (i != j) prod *= (xx - (float)(i + 1));
+    k[j] = prod;
+  }
+}
+
+// Interpolated value of one image plane at (px, py); blank outside the
+// image or where blanks carry more than a tenth of the weight.
+__device__ float beam_interp(const float *plane, const BeamImg &b, float px, float py, float fblank) {
+  if (px < 1.0f || px > (float)b.nx || py < 1.0f || py > (float)b.ny) return fblank;
+  float xk[9], yk[9];
+  int xs, ys;
+  lagrange(px, b.nx, b.hwidth, b.denom, &xs, xk);
+  lagrange(py, b.ny, b.hwidth, b.denom, &ys, yk);
+  const int iwid = 2 * b.hwidth + 1;
+  float sum = 0.0f, sumwt = 0.0f;
+  for (int j = 0; j < iwid; ++j) {
+    const float *row = plane + (long long)(ys + j) * b.nx + xs;
+    for (int i = 0; i < iwid; ++i) {
+      const float v = row[i];
+      if (v != fblank) {
+        const float wt = xk[i] * yk[j];
+        sumwt += wt;
+        sum += v * wt;
+      }
+    }
+  }
+  return (sumwt > 0.90f) ? sum / sumwt : fblank;
+}
+
+// 1/sqrt of a cos^2-shaped primary-beam gain (the reference's InvBeam,
+// ObitCUDABeamModel.cu:66-71); 100 beyond the first null.
+__device__ __forceinline__ float inv_beam(float fact, float freq, float ang) {
+  const float arg = fact * freq * ang;
+  if (arg > 1.02f) return 100.0f;
+  const float div = (float)(1.0 - 4.0 * ((double)arg * (double)arg));
+  return div / cosf((float)(3.141592653589793 * (double)arg));
+}
+
+// One thread per (component, antenna type), looping over beam channels.
+__global__ __launch_bounds__(256) void k_beam_jones(const float *__restrict__ images,
+                                                    const BeamImg *__restrict__ imgs,
+                                                    const float *__restrict__ pos,
+                                                    const float *__restrict__ bfreq, float refSizeFact,
+                                                    int ncomp, int nchan, float parAng, float fblank,
+                                                    float *__restrict__ jones) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y;
+  if (k >= ncomp) return;
+  const BeamImg b = imgs[t];
+  float *J = jones + ((long long)t * ncomp + k) * nchan * 8;
+  const double p0 = -(double)pos[2 * k], p1 = (double)pos[2 * k + 1];
+  float px, py;
+  if (beam_pixel(b, p0, p1, parAng, &px, &py) != 0) {
+    for (int c = 0; c < nchan; ++c) {
+      float *q = J + 8 * c;
+      q[0] = 1.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 0.0f;
+      q[4] = 0.0f; q[5] = 0.0f; q[6] = 1.0f; q[7] = 0.0f;
+    }
+    return;
+  }
+  const float ang = sqrtf((float)(p0 * p0 + p1 * p1));
+  const long long planeSz = (long long)b.nx * b.ny, imgSz = planeSz * nchan;
+  for (int c = 0; c < nchan; ++c) {
+    const float ipb = inv_beam(refSizeFact, bfreq[c], ang);
+    const float *base = images + b.off + (long long)c * planeSz;
+    // images in the reference's order: P1 re, P1 im, P2 re, P2 im,
+    // X1 re, X1 im, X2 re, X2 im
+    float P1r = ipb * beam_interp(base, b, px, py, fblank);
+    float P2r = ipb * beam_interp(base + 2 * imgSz, b, px, py, fblank);
+    float P1i, P2i, X1r, X1i, X2r, X2i;
+    // the reference's sanity test (it tests P1r twice)
+    if (ipb < 10.0f && fabsf(P1r) <= 1.3f && fabsf(P1r) <= 1.3f) {
+      P1i = ipb * beam_interp(base + 1 * imgSz, b, px, py, fblank);
+      P2i = ipb * beam_interp(base + 3 * imgSz, b, px, py, fblank);
+      X1r = ipb * beam_interp(base + 4 * imgSz, b, px, py, fblank);
+      X1i = ipb * beam_interp(base + 5 * imgSz, b, px, py, fblank);
+      X2r = ipb * beam_interp(base + 6 * imgSz, b, px, py, fblank);
+      X2i = ipb * beam_interp(base + 7 * imgSz, b, px, py, fblank);
+    } else {
+      P1r = P2r = 1.0f;
+      P1i = P2i = 0.0f;
+      X1r = X1i = X2r = X2i = 0.0f;
+    }
+    float *q = J + 8 * c;
+    q[0] = P1r; q[1] = P1i; q[2] = X1r; q[3] = X1i;
+    q[4] = X2r; q[5] = X2i; q[6] = P2r; q[7] = P2i;
+  }
+}
+
 }  // namespace beam
 }  // namespace obit
 
@@ -384,13 +539,18 @@ struct ObitHIPBeam {
   float *d_amp = nullptr, *d_g = nullptr;
   float *d_jones = nullptr;
   float2 *d_coh = nullptr;
+  // beam images (ObitHIPBeamDFTSetBeams)
+  float *d_images = nullptr, *d_bfreq = nullptr, *d_cpos = nullptr;
+  obit::beam::BeamImg *d_imgs = nullptr;
+  float refSizeFact = 0.0f, fblank = 0.0f;
 };
 
 namespace {
 
 void release(ObitHIPBeam *h) {
   if (!h) return;
-  void *ptrs[] = {h->d_freq, h->d_antType, h->d_jmap, h->d_pos, h->d_amp, h->d_g, h->d_jones, h->d_coh};
+  void *ptrs[] = {h->d_freq, h->d_antType, h->d_jmap, h->d_pos, h->d_amp, h->d_g, h->d_jones, h->d_coh,
+                  h->d_images, h->d_bfreq, h->d_cpos, h->d_imgs};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -548,6 +708,78 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   else
     fn = h->gauss ? (const void *)beam::k_beam_dft<4, beam::kCPT4, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, false>;
   OBIT_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(beam::kRows), args, 0, h->stream));
+  return kOK;
+}
+
+int ObitHIPBeamDFTGetJones(ObitHIPBeam *h, float *jones) {
+  using namespace obit;
+  if (!h || !jones) return fail(kInvalid, "ObitHIPBeamDFTGetJones: null argument");
+  const size_t n = (size_t)h->desc.ntype * std::max(h->desc.ncomp, 1) * h->desc.njchan;
+  OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
+  OBIT_HIP_TRY(hipMemcpy(jones, h->d_jones, n * 8 * sizeof(float), hipMemcpyDeviceToHost));
+  return kOK;
+}
+
+int ObitHIPBeamDFTSetBeams(ObitHIPBeam *h, const ObitHIPBeamImageDesc *desc, const float *images,
+                           const float *beamFreq, float refSizeFact, float fblank) {
+  using namespace obit;
+  if (!h || !desc || !images || !beamFreq) return fail(kInvalid, "ObitHIPBeamDFTSetBeams: null argument");
+  const int nt = h->desc.ntype, nch = h->desc.njchan;
+  std::vector<beam::BeamImg> bi(nt);
+  long long off = 0;
+  for (int t = 0; t < nt; ++t) {
+    const ObitHIPBeamImageDesc &d = desc[t];
+    if (d.nchan != nch)
+      return fail(kInvalid, "ObitHIPBeamDFTSetBeams: type %d has %d beam planes, the Jones table %d", t, d.nchan, nch);
+    if (d.hwidth < 1 || d.hwidth > 4) return fail(kInvalid, "ObitHIPBeamDFTSetBeams: hwidth must be 1..4");
+    if (d.nx < 2 * d.hwidth + 1 || d.ny < 2 * d.hwidth + 1)
+      return fail(kInvalid, "ObitHIPBeamDFTSetBeams: type %d image smaller than the interpolation kernel", t);
+    if (d.proj < 0 || d.proj > 3)
+      return fail(kInvalid, "ObitHIPBeamDFTSetBeams: projection %d not supported (linear, SIN, TAN, ARC)", d.proj);
+    beam::BeamImg &b = bi[t];
+    b.nx = d.nx; b.ny = d.ny; b.proj = d.proj; b.hwidth = d.hwidth;
+    b.crval[0] = d.crval[0]; b.crval[1] = d.crval[1];
+    b.crpix[0] = d.crpix[0]; b.crpix[1] = d.crpix[1];
+    b.cdelt[0] = d.cdelt[0]; b.cdelt[1] = d.cdelt[1];
+    const int iwid = 2 * d.hwidth + 1;
+    for (int j = 0; j < 10; ++j) b.denom[j] = 0.0f;
+    for (int j = 1; j <= iwid; ++j) {
+      float prod = 1.0f;
+      for (int i = 1; i <= iwid; ++i)
+        if (i != j) prod = prod * (float)(j - i);
+      b.denom[j - 1] = 1.0f / prod;
+    }
+    b.off = off;
+    off += 8LL * nch * d.nx * d.ny;
+  }
+  void *ptrs[] = {h->d_images, h->d_bfreq, h->d_imgs};
+  for (void *p : ptrs)
+    if (p) OBIT_HIP_TRY(hipFree(p));
+  h->d_images = h->d_bfreq = nullptr;
+  h->d_imgs = nullptr;
+  OBIT_HIP_TRY(hipMalloc(&h->d_images, (size_t)off * sizeof(float)));
+  OBIT_HIP_TRY(hipMemcpy(h->d_images, images, (size_t)off * sizeof(float), hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMalloc(&h->d_bfreq, (size_t)nch * sizeof(float)));
+  OBIT_HIP_TRY(hipMemcpy(h->d_bfreq, beamFreq, (size_t)nch * sizeof(float), hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMalloc(&h->d_imgs, (size_t)nt * sizeof(beam::BeamImg)));
+  OBIT_HIP_TRY(hipMemcpy(h->d_imgs, bi.data(), (size_t)nt * sizeof(beam::BeamImg), hipMemcpyHostToDevice));
+  h->refSizeFact = refSizeFact;
+  h->fblank = fblank;
+  return kOK;
+}
+
+int ObitHIPBeamDFTJonesFromBeams(ObitHIPBeam *h, const float *pos, float parAng) {
+  using namespace obit;
+  if (!h || (!pos && h->desc.ncomp > 0)) return fail(kInvalid, "ObitHIPBeamDFTJonesFromBeams: null argument");
+  if (!h->d_images) return fail(kState, "ObitHIPBeamDFTJonesFromBeams: no beam images (ObitHIPBeamDFTSetBeams)");
+  const int n = h->desc.ncomp;
+  if (n == 0) return kOK;
+  if (!h->d_cpos) OBIT_HIP_TRY(hipMalloc(&h->d_cpos, (size_t)n * 2 * sizeof(float)));
+  OBIT_HIP_TRY(hipMemcpy(h->d_cpos, pos, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(beam::k_beam_jones, dim3((unsigned)((n + 255) / 256), (unsigned)h->desc.ntype), dim3(256), 0,
+                     h->stream, h->d_images, h->d_imgs, h->d_cpos, h->d_bfreq, h->refSizeFact, n, h->desc.njchan,
+                     parAng, h->fblank, h->d_jones);
+  OBIT_HIP_TRY(hipGetLastError());
   return kOK;
 }
 

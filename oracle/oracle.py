@@ -7,6 +7,7 @@ package (obit_amd) never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -505,3 +506,124 @@ def beam_apply(buffer, model, live, nrparm, nif, nchan, incf, incif, ncorr, op="
                     buf[sel, q] -= mr[sel]
                     buf[sel, q + 1] -= mi[sel]
     return buf
+
+
+def beam_lagrange(p, n, hw):
+    """Start pixel (0-rel) and the 2 hw + 1 Lagrange weights for 1-rel
+    pixel p on an axis of n pixels (SetConvKernal, reference
+    include/CUDABeamInterp.h:233-258), FP32 as the device computes them."""
+    iwid = 2 * hw + 1
+    p = np.float32(p)
+    cen = int(p + np.float32(0.5)) - hw
+    cen = max(1, min(cen, n - iwid + 1)) - 1
+    xx = np.float32(p - np.float32(cen))
+    denom = []
+    for j in range(1, iwid + 1):
+        prod = np.float32(1.0)
+        for i in range(1, iwid + 1):
+            if i != j:
+                prod = np.float32(prod * np.float32(j - i))
+        denom.append(np.float32(1.0) / prod)
+    k = []
+    for j in range(iwid):
+        prod = denom[j]
+        for i in range(iwid):
+            if i != j:
+                prod = np.float32(prod * np.float32(xx - np.float32(i + 1)))
+        k.append(prod)
+    return cen, np.array(k, np.float32)
+
+
+def beam_interp(plane, px, py, hw, fblank):
+    """Interpolated value of an image plane [ny, nx] at 1-rel (px, py)
+    (CUDABeamInterpPixel, include/CUDABeamInterp.h:81-145): blank outside
+    the image or when the unblanked weights sum to 0.9 or less."""
+    ny, nx = plane.shape
+    if px < 1.0 or px > nx or py < 1.0 or py > ny:
+        return np.float32(fblank)
+    xs, xk = beam_lagrange(px, nx, hw)
+    ys, yk = beam_lagrange(py, ny, hw)
+    w = np.outer(yk.astype(np.float64), xk.astype(np.float64))
+    blk = plane[ys:ys + len(yk), xs:xs + len(xk)].astype(np.float64)
+    ok = blk != np.float32(fblank)
+    sw = float(np.sum(w[ok]))
+    if sw > 0.90:
+        return np.float32(np.sum(blk[ok] * w[ok]) / sw)
+    return np.float32(fblank)
+
+
+def beam_pixel(desc, x, y, rot):
+    """1-rel pixel of position (x, y) deg in a beam image rotated by rot
+    (CUDAPositionXYpix / CUDAPositionXYpixLM, include/CUDASkyGeom.h:90-312)
+    for linear, -SIN, -TAN and -ARC geometry; None where the projection
+    cannot place it."""
+    crval, crpix, cdelt, proj = desc["crval"], desc["crpix"], desc["cdelt"], desc.get("proj", 0)
+    dt = x - crval[0]
+    if dt > 180.0:
+        x -= 360.0
+    if dt < -180.0:
+        x += 360.0
+    d0, d1 = x - crval[0], y - crval[1]
+    if cdelt[0] == 0 or cdelt[1] == 0:
+        return None
+    if proj:
+        d2r = 1.745329252e-2
+        ra0, dec0, ra, dec = crval[0] * d2r, crval[1] * d2r, x * d2r, y * d2r
+        l = math.sin(ra - ra0) * math.cos(dec)
+        n = math.sin(dec) * math.sin(dec0) + math.cos(dec) * math.cos(dec0) * math.cos(ra - ra0)
+        m = math.sin(dec) * math.cos(dec0) - math.cos(dec) * math.sin(dec0) * math.cos(ra - ra0)
+        if proj == 1:
+            if n < 0:
+                return None
+        elif proj == 2:
+            if n <= 0:
+                return None
+            l, m = l / n, m / n
+        else:
+            c = math.acos(min(1.0, max(-1.0, n)))
+            c = c / math.sin(c) if c != 0 else 1.0
+            l, m = l * c, m * c
+        d0, d1 = l / d2r, m / d2r
+    rot = float(np.float32(rot))
+    sr, cr = float(np.float32(math.sin(rot))), float(np.float32(math.cos(rot)))
+    dz = d0 * cr + d1 * sr
+    d1 = d1 * cr - d0 * sr
+    return np.float32(dz / np.float32(cdelt[0]) + np.float32(crpix[0])), \
+        np.float32(d1 / np.float32(cdelt[1]) + np.float32(crpix[1]))
+
+
+def inv_beam(fact, freq, ang):
+    """1/sqrt of the cos^2 reference beam gain (InvBeam,
+    ObitCUDABeamModel.cu:66-71); 100 beyond 1.02 of the argument."""
+    arg = np.float32(np.float32(fact) * np.float32(freq) * np.float32(ang))
+    if arg > 1.02:
+        return np.float32(100.0)
+    div = np.float32(1.0 - 4.0 * float(arg) * float(arg))
+    return np.float32(div / np.float32(math.cos(float(np.float32(math.pi * float(arg))))))
+
+
+def beam_jones(descs, images, beam_freq, ref_size_fact, fblank, pos, par_ang):
+    """Jones table [ntype, ncomp, nchan, 2, 2] from beam images
+    [ntype][8][nchan, ny, nx] (GetJonesKernel, ObitCUDABeamModel.cu:1227-1308):
+    images in the order P1 re, P1 im, P2 re, P2 im, X1 re, X1 im, X2 re,
+    X2 im; pos [ncomp, 2] RA, Dec offsets (deg), placed at (-RA, Dec)."""
+    ntype, ncomp, nch = len(descs), len(pos), len(beam_freq)
+    J = np.zeros((ntype, ncomp, nch, 2, 2), np.complex128)
+    J[..., 0, 0] = J[..., 1, 1] = 1.0
+    for t in range(ntype):
+        d, img = descs[t], np.asarray(images[t], np.float32)
+        hw = d.get("hwidth", 2)
+        for k in range(ncomp):
+            x, y = -float(np.float32(pos[k][0])), float(np.float32(pos[k][1]))
+            pix = beam_pixel(d, x, y, par_ang)
+            if pix is None:
+                continue
+            ang = np.float32(math.sqrt(np.float32(x * x + y * y)))
+            for c in range(nch):
+                ipb = inv_beam(ref_size_fact, beam_freq[c], ang)
+                val = lambda i: np.float32(ipb * beam_interp(img[i, c], pix[0], pix[1], hw, fblank))
+                p1r, p2r = val(0), val(2)
+                if ipb < 10.0 and abs(p1r) <= 1.3:
+                    J[t, k, c] = [[p1r + 1j * val(1), val(4) + 1j * val(5)],
+                                  [val(6) + 1j * val(7), p2r + 1j * val(3)]]
+    return J

@@ -145,3 +145,82 @@ def test_beam_refuses_bad_tables():
         BeamDFT(uv, comps, np.full(NANT, 3), np.zeros(8, int), 1, ntype=2)
     with pytest.raises(Exception, match="beam channel"):
         BeamDFT(uv, comps, np.zeros(NANT, int), np.full(8, 4), 2)
+
+
+FBLANK = -1.0e30
+
+
+def _beams(ntype, nch, proj, nx=48, ny=40):
+    """Smooth beam images with a blanked patch: P1, P2 real parts a
+    Gaussian bump (<= 0.95), the other six small ripples."""
+    y, x = np.mgrid[0:ny, 0:nx] / 10.0
+    images = []
+    for t in range(ntype):
+        imgs = np.zeros((8, nch, ny, nx), np.float32)
+        for c in range(nch):
+            bump = np.exp(-((x - nx / 20.0) ** 2 + (y - ny / 20.0) ** 2) / (6.0 + c + 0.5 * t))
+            imgs[0, c] = 0.95 * bump
+            imgs[2, c] = 0.9 * bump
+            for i in (1, 3, 4, 5, 6, 7):
+                imgs[i, c] = 0.05 * np.sin(0.3 * (i + 1) * x + 0.2 * t * y + c)
+        imgs[:, :, 3:6, 7:10] = FBLANK
+        images.append(imgs)
+    descs = [dict(nx=nx, ny=ny, proj=proj, hwidth=2, crval=(0.0, 0.0), crpix=(nx / 2 + 0.5, ny / 2 + 0.5),
+                  cdelt=(-0.1, 0.1)) for _ in range(ntype)]
+    return descs, images
+
+
+def _positions(n, seed=9):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1.8, 1.8, (n, 2)).astype(np.float32)
+    pos[0] = (9.0, 9.0)          # off the images: unit matrices
+    pos[1] = (2.2, -1.4)         # on the blanked patch
+    return pos
+
+
+@pytest.mark.parametrize("proj", [0, 1])
+def test_jones_from_beams_matches_oracle(proj):
+    uv = _data(2, rows=64, nif=1, nchan=24)
+    comps = _comps(True)
+    nch = 3
+    descs, images = _beams(2, nch, proj)
+    bfreq = np.array([1.3e9, 1.4e9, 1.5e9])
+    fact = 2.0e-10
+    pos = _positions(len(comps))
+    ant_type = np.arange(NANT) % 2
+    jmap = np.arange(uv.nchan) // 8
+    with BeamDFT(uv, comps, ant_type, jmap, nch, ntype=2) as bm:
+        bm.set_beams(descs, images, bfreq, fact, FBLANK)
+        bm.jones_from_beams(pos, par_ang=0.3)
+        got = bm.get_jones().astype(np.complex128)
+    want = orc.beam_jones(descs, images, bfreq, fact, FBLANK, pos, 0.3)
+    assert np.array_equal(got[:, 0], want[:, 0])                        # unit matrices
+    assert np.abs(got - want).max() < 2e-5, np.abs(got - want).max()
+    assert np.abs(want - np.eye(2)).max() > 0.1                          # the beams do act
+
+
+def test_beam_chain_from_images_matches_oracle():
+    """SetBeams + JonesFromBeams + Calc against the oracle's Jones table,
+    model and application."""
+    uv = _data(4, rows=800, nif=1, nchan=24, seed=21)
+    comps = _comps(False)
+    nch = 3
+    descs, images = _beams(2, nch, 1)
+    bfreq = np.array([1.3e9, 1.4e9, 1.5e9])
+    pos = _positions(len(comps))
+    ant_type = np.arange(NANT) % 2
+    jmap = np.arange(uv.nchan) // 8
+    chi = -0.6
+    with BeamDFT(uv, comps, ant_type, jmap, nch, ntype=2, stok_type=2, op="sub") as bm:
+        bm.set_beams(descs, images, bfreq, 2.0e-10, FBLANK)
+        bm.jones_from_beams(pos, par_ang=chi)
+        got = bm.calc(uv.buffer, par_ang=chi)
+    J = orc.beam_jones(descs, images, bfreq, 2.0e-10, FBLANK, pos, chi)
+    model, live = orc.beam_model_fp64(uv.buffer, comps, uv.freqArr / uv.freq, ant_type, jmap, J, 2, chi,
+                                      ilocb=uv.ilocb)
+    want = orc.beam_apply(uv.buffer, model, live, uv.nrparm, uv.nif, uv.nchan, uv.incf, uv.incif, 4, "sub")
+    moved = np.isnan(orc.beam_apply(uv.buffer, np.full_like(model, complex(np.nan, np.nan)), live, uv.nrparm,
+                                    uv.nif, uv.nchan, uv.incf, uv.incif, 4, "sub"))
+    assert np.array_equal(got[~moved], uv.buffer[~moved])
+    peak = np.abs(model[live]).max()
+    assert np.abs(got[moved].astype(np.float64) - want[moved]).max() < 1e-5 * peak
