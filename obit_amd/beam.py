@@ -3,8 +3,9 @@ through ObitHIPBeamDFT* (include/ObitHIPBeam.h).
 
 Drives the layer the way Obit's beam-model host class drives its CUDA
 layer (reference src/ObitGPUBeamModel.c: Init, SetMod, Calc per
-parallactic angle, Shutdown; src/ObitCUDABeamModel.cu:1368-1437), with the
-Jones table handed in instead of interpolated from beam images.
+parallactic angle, Shutdown; src/ObitCUDABeamModel.cu:1368-1437).  The
+Jones table is handed in, set to unit matrices, or formed on the device
+from beam images (GetJonesKernel, ObitCUDABeamModel.cu:1227-1308).
 """
 from __future__ import annotations
 
