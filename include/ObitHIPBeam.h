@@ -95,6 +95,16 @@ int ObitHIPBeamDFTSetJones(ObitHIPBeam *h, const float *jones);
  * parAng (radians) (ObitCUDABeamModelDFTCalc).  Asynchronous on the
  * handle's stream. */
 int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAng);
+/* Per-channel amplitudes [ncomp][nif*nchan] (IF-major) that replace the
+ * table's amplitude column, for the spectral component types: the
+ * reference's beamPointSpec / beamGaussSpec kernels form
+ * a exp(-sum_t c_t (-ln(nu/nu_ref))^(t+1)) and beamPointTSpec /
+ * beamGaussTSpec flux_sb (nu/nu_sb)^SI_sb per subband
+ * (ObitCUDABeamModel.cu:312-505, 506-686, 860-1226); they depend on the
+ * component and the channel only, so they are formed once, in FP64, by
+ * the caller.  A Gaussian's factor still multiplies them.  NULL returns to
+ * the amplitude column. */
+int ObitHIPBeamDFTSetAmps(ObitHIPBeam *h, const float *amp);
 /* Copy the current Jones table to the host ([ntype][ncomp][njchan][8]). */
 int ObitHIPBeamDFTGetJones(ObitHIPBeam *h, float *jones);
 

@@ -168,6 +168,7 @@ _PROTOS = {
     "ObitHIPBeamDFTSetMod": (_I, [_P, _P]),
     "ObitHIPBeamDFTSetJones": (_I, [_P, _P]),
     "ObitHIPBeamDFTCalc": (_I, [_P, _P, _LL, _F]),
+    "ObitHIPBeamDFTSetAmps": (_I, [_P, _P]),
     "ObitHIPBeamDFTGetJones": (_I, [_P, _P]),
     "ObitHIPBeamDFTSetBeams": (_I, [_P, _P, _P, _P, _F, _F]),
     "ObitHIPBeamDFTJonesFromBeams": (_I, [_P, _P, _F]),

@@ -18,6 +18,30 @@ from . import _lib
 OPS = {"sub": 0, "div": 1, "repl": 2}
 
 
+def tspec_amps(flux, si, spec_index, freq_scale, freq_rat):
+    """Per-channel amplitudes of tabulated spectra as the reference's
+    beamPointTSpec / beamGaussTSpec kernels form them
+    (ObitCUDABeamModel.cu:559-587): flux_sb exp(SI_sb ln(nu/nu0 * nu0/nu_sb))
+    for the subband sb = spec_index[channel], in FP64.
+    flux, si [ncomp, nspec]; spec_index, freq_scale [nif*nchan];
+    freq_rat [nspec] = nu0 / nu_sb."""
+    flux, si = np.asarray(flux, np.float64), np.asarray(si, np.float64)
+    sb = np.asarray(spec_index)
+    lnr = np.log(np.asarray(freq_scale, np.float64) * np.asarray(freq_rat, np.float64)[sb])
+    return (flux[:, sb] * np.exp(si[:, sb] * lnr[None, :])).astype(np.float32)
+
+
+def pspec_amps(amp, coef, freq_scale, freq_rat):
+    """Per-channel amplitudes of a parameterised spectrum as the reference's
+    beamPointSpec / beamGaussSpec kernels form them
+    (ObitCUDABeamModel.cu:367-398): a exp(-sum_t c_t L^(t+1)) with
+    L = -ln(nu/nu0 * freq_rat), in FP64.  amp [ncomp], coef [ncomp, nterm]."""
+    L = -np.log(np.asarray(freq_scale, np.float64) * float(freq_rat))
+    coef = np.asarray(coef, np.float64)
+    arg = sum(coef[:, t:t + 1] * L[None, :] ** (t + 1) for t in range(coef.shape[1]))
+    return (np.asarray(amp, np.float64)[:, None] * np.exp(-arg)).astype(np.float32)
+
+
 def freq_lines(uv):
     """nu/nu0 of channel 0 of each IF and the per-channel step, from the
     IF-major frequency table (channels are evenly spaced within an IF)."""
@@ -68,6 +92,15 @@ class BeamDFT:
         j = np.asarray(jones, np.complex64)
         flat = np.ascontiguousarray(j.reshape(j.shape[:3] + (4,)).view(np.float32))
         _lib.check(self.lib.ObitHIPBeamDFTSetJones(self.h, flat.ctypes.data), "ObitHIPBeamDFTSetJones")
+
+    def set_amps(self, amps=None):
+        """Per-channel amplitudes [ncomp, nif*nchan] (spectral types), or
+        None for the table's amplitude column."""
+        if amps is None:
+            _lib.check(self.lib.ObitHIPBeamDFTSetAmps(self.h, None), "ObitHIPBeamDFTSetAmps")
+            return
+        a = np.ascontiguousarray(amps, np.float32)
+        _lib.check(self.lib.ObitHIPBeamDFTSetAmps(self.h, a.ctypes.data), "ObitHIPBeamDFTSetAmps")
 
     def get_jones(self):
         """The device's Jones table as complex [ntype, ncomp, njchan, 2, 2]."""

@@ -46,7 +46,7 @@ struct Desc {
   float *vis;
   long long nrow;
   int lrec, nrparm, ilocu, ilocv, ilocw, ilocb, iloca1, iloca2;
-  int nchan, incf, incif;
+  int nif, nchan, incf, incif;
   int nchunk, nrowblk;
   long long nitems;
   int ncomp, ntype, nant, njchan, npair;
@@ -56,6 +56,7 @@ struct Desc {
   const float *cg;             // Gaussian coefficients [3][ncomp]
   const int *antType, *jmap;
   const float2 *coh;           // [pair][comp][jchan][4]: XX, XY, YX, YY
+  const float *tab;            // per-channel amplitudes [comp][nif * nchan], or null
   int op;
   float sf[4];                 // XX, XY, YX, YY
 };
@@ -148,11 +149,12 @@ __device__ __forceinline__ int prod_index(int e) {
   return NC == 2 ? 3 * e : e;
 }
 
-template <int NC, int kCPT, bool GAUSS>
+template <int NC, int kCPT, bool GAUSS, bool SPEC>
 __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
   __shared__ double s_x[kTile], s_y[kTile], s_z[kTile];
   __shared__ float s_a[kTile];
   __shared__ float s_g[GAUSS ? 3 * kTile : 1];
+  __shared__ float s_t[SPEC ? kTile * kCPT : 1];
   __shared__ float2 s_m[kTile * kMaxPairs * NC];
   const int tid = threadIdx.x;
 
@@ -212,6 +214,14 @@ __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
           s_g[3 * t + 2] = ok ? d.cg[2 * d.ncomp + k] : 0.0f;
         }
       }
+      if (SPEC) {
+        const long long ld = (long long)d.nif * d.nchan;
+        for (int t = tid; t < kTile * kCPT; t += kRows) {
+          const int kk = t / kCPT, cc = t - kk * kCPT;
+          const int k = k0 + kk, ch = c0 + cc;
+          s_t[t] = (k < d.ncomp && ch < d.nchan) ? d.tab[(long long)k * ld + (long long)iIF * d.nchan + ch] : 0.0f;
+        }
+      }
       if (useLds) {
         const int per = d.npair * NC;
         for (int t = tid; t < kTile * per; t += kRows) {
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
           const float e0 = G * fmf * fmf;
           const float Gd = G * fdf;
           const bool on = e0 > -60.0f;
-          gf = on ? a0 * __expf(e0) : 0.0f;
+          gf = on ? (SPEC ? 1.0f : a0) * __expf(e0) : 0.0f;
           rf = on ? sgn * __expf(Gd * (2.0f * fmf + fdf)) : 0.0f;
           rbk = on ? sgn * __expf(Gd * (fdf - 2.0f * fmf)) : 0.0f;
           qg = on ? __expf(2.0f * Gd * fdf) : 0.0f;
@@ -268,7 +278,9 @@ __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
         for (int j = kCPT / 2; j < kCPT; ++j) {
           const bool odd = (j - kCPT / 2) & 1;
           {
-            const float amp = GAUSS ? g : (odd ? ao : a0);
+            const float amp = GAUSS ? (SPEC ? s_t[kk * kCPT + j] * g : g)
+                                    : (SPEC ? (odd ? s_t[kk * kCPT + j] * sgn : s_t[kk * kCPT + j])
+                                            : (odd ? ao : a0));
             const float vr = amp * x, vi = amp * y;
 #pragma unroll
             for (int e = 0; e < NC; ++e) {
@@ -293,7 +305,9 @@ __global__ __launch_bounds__(kRows) void k_beam_dft(Desc d) {
           y = fmaf(-sig, x1, y);
           x = fmaf(tau, y, x1);
           {
-            const float amp = GAUSS ? gb : (odd ? ao : a0);
+            const float amp = GAUSS ? (SPEC ? s_t[kk * kCPT + j] * gb : gb)
+                                    : (SPEC ? (odd ? s_t[kk * kCPT + j] * sgn : s_t[kk * kCPT + j])
+                                            : (odd ? ao : a0));
             const float vr = amp * x, vi = amp * y;
 #pragma unroll
             for (int e = 0; e < NC; ++e) {
@@ -532,7 +546,7 @@ struct ObitHIPBeam {
   ObitHIPBeamDesc desc;
   hipStream_t stream = nullptr;
   int ncu = 0;
-  bool gauss = false, anyFlux = false, haveMod = false;
+  bool gauss = false, anyFlux = false, anyTab = false, haveMod = false;
   double *d_freq = nullptr;   // fIF[nif], dfIF[nif]
   int *d_antType = nullptr, *d_jmap = nullptr;
   double *d_pos = nullptr;    // cx, cy, cz [3][ncomp]
@@ -541,6 +555,7 @@ struct ObitHIPBeam {
   float2 *d_coh = nullptr;
   // beam images (ObitHIPBeamDFTSetBeams)
   float *d_images = nullptr, *d_bfreq = nullptr, *d_cpos = nullptr;
+  float *d_tab = nullptr;     // per-channel amplitudes (ObitHIPBeamDFTSetAmps)
   obit::beam::BeamImg *d_imgs = nullptr;
   float refSizeFact = 0.0f, fblank = 0.0f;
 };
@@ -550,7 +565,7 @@ namespace {
 void release(ObitHIPBeam *h) {
   if (!h) return;
   void *ptrs[] = {h->d_freq, h->d_antType, h->d_jmap, h->d_pos, h->d_amp, h->d_g, h->d_jones, h->d_coh,
-                  h->d_images, h->d_bfreq, h->d_cpos, h->d_imgs};
+                  h->d_images, h->d_bfreq, h->d_cpos, h->d_imgs, h->d_tab};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -671,7 +686,7 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   const ObitHIPBeamDesc &D = h->desc;
   // no component with flux: the data are left as they are (the reference
   // kernels' gotSome)
-  if (nrow == 0 || D.ncomp == 0 || !h->anyFlux) return kOK;
+  if (nrow == 0 || D.ncomp == 0 || !(h->d_tab ? h->anyTab : h->anyFlux)) return kOK;
   const double c2 = std::cos(2.0 * (double)parAng), s2 = std::sin(2.0 * (double)parAng);
   const long long ncoh = (long long)D.ntype * D.ntype * D.ncomp * D.njchan;
   hipLaunchKernelGGL(beam::k_beam_coh, dim3((unsigned)((ncoh + 255) / 256)), dim3(256), 0, h->stream,
@@ -683,7 +698,7 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   d.lrec = D.lrec; d.nrparm = D.nrparm;
   d.ilocu = D.ilocu; d.ilocv = D.ilocv; d.ilocw = D.ilocw;
   d.ilocb = D.ilocb; d.iloca1 = D.iloca1; d.iloca2 = D.iloca2;
-  d.nchan = D.nchan; d.incf = D.incf; d.incif = D.incif;
+  d.nif = D.nif; d.nchan = D.nchan; d.incf = D.incf; d.incif = D.incif;
   const int cpt = D.ncorr == 2 ? beam::kCPT2 : beam::kCPT4;
   d.nchunk = (D.nchan + cpt - 1) / cpt;
   const long long nrowblk = (nrow + beam::kRows - 1) / beam::kRows;
@@ -697,17 +712,40 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   d.camp = h->d_amp; d.cg = h->d_g;
   d.antType = h->d_antType; d.jmap = h->d_jmap;
   d.coh = h->d_coh;
+  d.tab = h->d_tab;
   d.op = D.opType;
   for (int i = 0; i < 4; ++i) d.sf[i] = D.stokFact[i];
   long long grid = (long long)std::max(h->ncu, 1) * 2;
   if (grid > d.nitems) grid = d.nitems;
   void *args[] = {&d};
   const void *fn;
-  if (D.ncorr == 2)
-    fn = h->gauss ? (const void *)beam::k_beam_dft<2, beam::kCPT2, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, false>;
-  else
-    fn = h->gauss ? (const void *)beam::k_beam_dft<4, beam::kCPT4, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, false>;
+  const bool sp = h->d_tab != nullptr;
+  if (D.ncorr == 2) {
+    if (h->gauss) fn = sp ? (const void *)beam::k_beam_dft<2, beam::kCPT2, true, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, true, false>;
+    else fn = sp ? (const void *)beam::k_beam_dft<2, beam::kCPT2, false, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, false, false>;
+  } else {
+    if (h->gauss) fn = sp ? (const void *)beam::k_beam_dft<4, beam::kCPT4, true, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, true, false>;
+    else fn = sp ? (const void *)beam::k_beam_dft<4, beam::kCPT4, false, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, false, false>;
+  }
   OBIT_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(beam::kRows), args, 0, h->stream));
+  return kOK;
+}
+
+int ObitHIPBeamDFTSetAmps(ObitHIPBeam *h, const float *amp) {
+  using namespace obit;
+  if (!h) return fail(kInvalid, "ObitHIPBeamDFTSetAmps: null handle");
+  OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
+  if (!amp) {
+    if (h->d_tab) OBIT_HIP_TRY(hipFree(h->d_tab));
+    h->d_tab = nullptr;
+    return kOK;
+  }
+  const size_t n = (size_t)std::max(h->desc.ncomp, 1) * h->desc.nif * h->desc.nchan;
+  if (!h->d_tab) OBIT_HIP_TRY(hipMalloc(&h->d_tab, n * sizeof(float)));
+  OBIT_HIP_TRY(hipMemcpy(h->d_tab, amp, n * sizeof(float), hipMemcpyHostToDevice));
+  bool any = false;
+  for (size_t i = 0; i < n && !any; ++i) any = amp[i] != 0.0f;
+  h->anyTab = any;
   return kOK;
 }
 

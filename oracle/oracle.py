@@ -429,14 +429,16 @@ def beam_products(jones, ant_types_pairs, stok_type, par_ang):
 
 
 def beam_model_fp64(buffer, comps, freq_scale, ant_type, vis_jchan, jones, stok_type=1, par_ang=0.0,
-                    ilocu=0, ilocv=1, ilocw=2, ilocb=3, iloca1=-1, iloca2=-1, stok_fact=(1, 1, 1, 1)):
+                    ilocu=0, ilocv=1, ilocw=2, ilocb=3, iloca1=-1, iloca2=-1, stok_fact=(1, 1, 1, 1),
+                    amps=None):
     """FP64 model [nrow, nif*nchan, 2, 2] (XX XY / YX YY) of the
     Jones-corrected DFT, and the mask of rows whose antennas are in the
     type table.  Phase u x + v y + w z scaled by nu/nu0 (radians, x in
     radians per wavelength); a Gaussian multiplies the amplitude by
     exp((nu/nu0)^2 (g0 u^2 + g1 v^2 + g2 uv)) (reference
     src/ObitCUDABeamModel.cu:197-203, 761-767).  freq_scale: nu/nu0 per
-    (IF, channel), IF-major."""
+    (IF, channel), IF-major.  amps [ncomp, nif*nchan], when given, replaces
+    the amplitude column per channel (spectral component types)."""
     buf = np.asarray(buffer, np.float32)
     comps = np.asarray(comps, np.float32).astype(np.float64)
     a1, a2 = beam_ants(buf, ilocb, iloca1, iloca2)
@@ -458,7 +460,8 @@ def beam_model_fp64(buffer, comps, freq_scale, ant_type, vis_jchan, jones, stok_
     rows_of = {p: np.nonzero((t1 == p[0]) & (t2 == p[1]))[0] for p in pairs}
     for f in range(nf):
         fs = float(freq_scale[f])
-        amp = comps[:, 0][None, :] * (np.exp(fs * fs * G) if gauss else 1.0)
+        a0 = (np.asarray(amps, np.float32)[:, f].astype(np.float64) if amps is not None else comps[:, 0])
+        amp = a0[None, :] * (np.exp(fs * fs * G) if gauss else 1.0)
         V = amp * np.exp(1j * fs * psi)                       # [nrow, ncomp]
         jc = int(vis_jchan[f])
         for p, r in rows_of.items():

@@ -224,3 +224,43 @@ def test_beam_chain_from_images_matches_oracle():
     assert np.array_equal(got[~moved], uv.buffer[~moved])
     peak = np.abs(model[live]).max()
     assert np.abs(got[moved].astype(np.float64) - want[moved]).max() < 1e-5 * peak
+
+
+@pytest.mark.parametrize("ncorr,gauss,uniform,op", [(2, False, True, "sub"), (4, True, False, "repl"),
+                                                    (2, True, True, "div")])
+def test_beam_spectral_table_matches_oracle(ncorr, gauss, uniform, op):
+    """Tabulated spectra (flux and spectral index per subband, the
+    reference's TSpec form) through ObitHIPBeamDFTSetAmps."""
+    from obit_amd.beam import tspec_amps
+    uv = _data(ncorr, rows=1000, seed=31 + ncorr)
+    comps = _comps(gauss)
+    rng = np.random.default_rng(4)
+    nf = uv.nif * uv.nchan
+    nspec = 4
+    spec_index = (np.arange(nf) * nspec) // nf
+    fs = uv.freqArr / uv.freq
+    freq_rat = 1.0 / np.array([fs[spec_index == s].mean() for s in range(nspec)])
+    flux = np.abs(comps[:, :1]) * rng.uniform(0.5, 1.5, (len(comps), nspec))
+    flux[0] = 1.0
+    si = rng.normal(-0.7, 0.3, (len(comps), nspec))
+    amps = tspec_amps(flux, si, spec_index, fs, freq_rat)
+    ant_type = rng.integers(0, 2, NANT)
+    ch = np.arange(nf) % uv.nchan
+    jmap = (np.arange(nf) // uv.nchan) * 5 + (ch // 8 if uniform else ch // 6)
+    njchan = int(jmap.max()) + 1
+    jones = _jones(2, len(comps), njchan, seed=6)
+    with BeamDFT(uv, comps, ant_type, jmap, njchan, ntype=2, stok_type=1, op=op) as bm:
+        bm.set_jones(jones)
+        bm.set_amps(amps)
+        got = bm.calc(uv.buffer, par_ang=0.2)
+    model, live = orc.beam_model_fp64(uv.buffer, comps, fs, ant_type, jmap, jones, 1, 0.2, ilocb=uv.ilocb,
+                                      amps=amps)
+    want = orc.beam_apply(uv.buffer, model, live, uv.nrparm, uv.nif, uv.nchan, uv.incf, uv.incif, ncorr, op)
+    moved = np.isnan(orc.beam_apply(uv.buffer, np.full_like(model, complex(np.nan, np.nan)), live, uv.nrparm,
+                                    uv.nif, uv.nchan, uv.incf, uv.incif, ncorr, op))
+    assert np.array_equal(got[~moved], uv.buffer[~moved])
+    err = np.abs(got[moved].astype(np.float64) - want[moved]).max()
+    if op == "div":
+        assert err < 2e-5 * np.abs(want[moved]).max(), err
+    else:
+        assert err < 1e-5 * np.abs(model[live]).max(), err

@@ -209,3 +209,42 @@ def test_apply_rules():
     re, im = np.float32(2.0), np.float32(0.5)
     assert div[0, nrparm + 3] == (mr / wm) * re + (mi / wm) * im
     assert div[0, nrparm + 5] == np.float32(1.0) * np.sqrt(wm)
+
+
+def test_spectral_amplitude_helpers():
+    """Tabulated spectra: the subband flux at its own frequency, a power law
+    in between, SI = 0 flat; parameterised: one term is a power law."""
+    from obit_amd.beam import pspec_amps, tspec_amps
+    fs = np.array([0.9, 1.0, 1.1, 1.25])
+    spec_index = np.array([0, 0, 1, 1])
+    freq_rat = np.array([1.0 / 0.95, 1.0 / 1.2])        # nu0 / nu_sb
+    flux = np.array([[2.0, 1.0], [0.5, 0.25]])
+    si = np.array([[-0.7, 0.3], [0.0, 0.0]])
+    a = tspec_amps(flux, si, spec_index, fs, freq_rat)
+    for f in range(4):
+        sb = spec_index[f]
+        assert abs(a[0, f] - flux[0, sb] * (fs[f] / (1.0 / freq_rat[sb])) ** si[0, sb]) < 1e-6 * flux[0, sb]
+        assert a[1, f] == np.float32(flux[1, sb])
+    p = pspec_amps(np.array([3.0]), np.array([[0.8]]), fs, 1.0)
+    assert np.allclose(p[0], 3.0 * fs ** 0.8, rtol=1e-6)
+    p2 = pspec_amps(np.array([1.0]), np.array([[0.0, 0.5]]), fs, 1.0)
+    assert np.allclose(p2[0], np.exp(-0.5 * np.log(fs) ** 2), rtol=1e-6)
+
+
+def test_amplitude_table_replaces_the_column():
+    """A table that repeats the column gives the same model; a table
+    scaled per channel scales the model per channel."""
+    rng = np.random.default_rng(8)
+    uv, comps = _small(rng, rows=12, ncomp=5)
+    nf = uv.nif * uv.nchan
+    at = np.zeros(27, int)
+    jm = np.zeros(nf, int)
+    jones = _jones(rng, 1, 5, 1, scale=0.2)
+    m0, _ = orc.beam_model_fp64(uv.buffer, comps, uv.freqArr / uv.freq, at, jm, jones, 1, 0.0)
+    tab = np.repeat(comps[:, :1], nf, axis=1)
+    m1, _ = orc.beam_model_fp64(uv.buffer, comps, uv.freqArr / uv.freq, at, jm, jones, 1, 0.0, amps=tab)
+    assert np.abs(m1 - m0).max() < 1e-12 * np.abs(m0).max()
+    scale = np.linspace(0.5, 2.0, nf)
+    m2, _ = orc.beam_model_fp64(uv.buffer, comps, uv.freqArr / uv.freq, at, jm, jones, 1, 0.0,
+                                amps=(tab * scale[None, :]).astype(np.float32))
+    assert np.abs(m2 - m0 * scale[None, :, None, None]).max() < 1e-6 * np.abs(m0).max()
