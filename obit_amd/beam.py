@@ -57,7 +57,7 @@ class BeamDFT:
     """One data layout, component table and Jones table on the device."""
 
     def __init__(self, uv, comps, ant_type, vis_jchan, njchan, ntype=None, stok_type=1, op="sub",
-                 stok_fact=(1.0, 1.0, 1.0, 1.0), ncorr=None, stream=None):
+                 stok_fact=(1.0, 1.0, 1.0, 1.0), ncorr=None, stream=None, iloca=None):
         self.lib = _lib.load()
         self.uv = uv
         comps = np.ascontiguousarray(comps, np.float32)
@@ -66,7 +66,8 @@ class BeamDFT:
         d = _lib.BeamDesc()
         d.lrec, d.nrparm = uv.lrec, uv.nrparm
         d.ilocu, d.ilocv, d.ilocw = uv.ilocu, uv.ilocv, uv.ilocw
-        d.ilocb, d.iloca1, d.iloca2 = uv.ilocb, -1, -1
+        # iloca: (ANTENNA1, ANTENNA2) parameter offsets for data without BASELINE
+        d.ilocb, d.iloca1, d.iloca2 = (uv.ilocb, -1, -1) if iloca is None else (-1, iloca[0], iloca[1])
         d.nif, d.nchan, d.incf, d.incif = uv.nif, uv.nchan, uv.incf, uv.incif
         d.ncorr = int(ncorr if ncorr is not None else uv.nstok)
         d.ncomp, d.compLen = comps.shape

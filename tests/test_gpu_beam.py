@@ -264,3 +264,30 @@ def test_beam_spectral_table_matches_oracle(ncorr, gauss, uniform, op):
         assert err < 2e-5 * np.abs(want[moved]).max(), err
     else:
         assert err < 1e-5 * np.abs(model[live]).max(), err
+
+
+def test_beam_antenna_parameters():
+    """Rows with ANTENNA1/ANTENNA2 parameters instead of BASELINE give the
+    same result as the BASELINE rows they were made from."""
+    uv = _data(2, rows=600, nif=1, nchan=16, seed=41)
+    cb = uv.buffer[:, uv.ilocb].astype(np.int64)
+    buf2 = np.zeros((uv.nrow, uv.lrec + 2), np.float32)
+    buf2[:, :uv.nrparm] = uv.buffer[:, :uv.nrparm]
+    buf2[:, uv.nrparm] = (cb // 256).astype(np.float32)          # ANTENNA1
+    buf2[:, uv.nrparm + 1] = (cb % 256).astype(np.float32)       # ANTENNA2
+    buf2[:, uv.nrparm + 2:] = uv.buffer[:, uv.nrparm:]
+    buf2[:, uv.ilocb] = 0.0                                      # no BASELINE
+    uv2 = synth.UVData(buffer=buf2, nchan=uv.nchan, nif=uv.nif, nstok=2, freq=uv.freq, freqArr=uv.freqArr,
+                       crval_stokes=-5.0, nrparm=uv.nrparm + 2)
+    comps = _comps(True)
+    rng = np.random.default_rng(2)
+    at = rng.integers(0, 2, NANT)
+    jones = _jones(2, len(comps), 2)
+    jmap = np.arange(uv.nchan) // 8
+    with BeamDFT(uv, comps, at, jmap, 2, ntype=2, stok_type=3) as bm:
+        bm.set_jones(jones)
+        ref = bm.calc(uv.buffer, par_ang=0.5)
+    with BeamDFT(uv2, comps, at, jmap, 2, ntype=2, stok_type=3, iloca=(uv.nrparm, uv.nrparm + 1)) as bm:
+        bm.set_jones(jones)
+        got = bm.calc(buf2, par_ang=0.5)
+    assert np.array_equal(got[:, uv.nrparm + 2:], ref[:, uv.nrparm:])
