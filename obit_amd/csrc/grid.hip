@@ -252,8 +252,10 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
   const long long total = nrow * (long long)nf;
   for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
        it += (long long)gridDim.x * blockDim.x) {
-    const long long r = it % nrow;
-    const int fi = (int)(it / nrow);
+    // facet fastest: a wave's lanes read one or two rows and their
+    // counter atomics fall on different facets' lists
+    const int fi = (int)(it % nf);
+    const long long r = it / nf;
     const FacetDev &f = facets[fi];
     const float *row = vis + r * (long long)cm.lenvis;
     float uu, vv, ww;
