@@ -140,6 +140,9 @@ int ObitHIPBeamDFTSetBeams(ObitHIPBeam *h, const ObitHIPBeamImageDesc *desc, con
  * Asynchronous on the handle's stream. */
 int ObitHIPBeamDFTJonesFromBeams(ObitHIPBeam *h, const float *pos, float parAng);
 
+/* The Set* calls and JonesFromBeams first wait for the handle's stream,
+ * so tables are never replaced under a running Calc. */
+
 /* Release the handle (ObitCUDABeamModelDFTShutdown). */
 int ObitHIPBeamDFTDestroy(ObitHIPBeam *h);
 

@@ -639,6 +639,8 @@ int ObitHIPBeamDFTCreate(ObitHIPBeam **out, const ObitHIPBeamDesc *desc, const d
 int ObitHIPBeamDFTSetMod(ObitHIPBeam *h, const float *comps) {
   using namespace obit;
   if (!h || (!comps && h->desc.ncomp > 0)) return fail(kInvalid, "ObitHIPBeamDFTSetMod: null argument");
+  // a previous call's kernels may still read the tables being replaced
+  OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
   const int n = h->desc.ncomp, L = h->desc.compLen;
   std::vector<double> pos(3 * (size_t)std::max(n, 1));
   std::vector<float> amp(std::max(n, 1)), g(3 * (size_t)std::max(n, 1));
@@ -668,6 +670,7 @@ int ObitHIPBeamDFTSetMod(ObitHIPBeam *h, const float *comps) {
 int ObitHIPBeamDFTSetJones(ObitHIPBeam *h, const float *jones) {
   using namespace obit;
   if (!h) return fail(kInvalid, "ObitHIPBeamDFTSetJones: null handle");
+  OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
   const size_t n = (size_t)h->desc.ntype * std::max(h->desc.ncomp, 1) * h->desc.njchan;
   if (jones) {
     OBIT_HIP_TRY(hipMemcpy(h->d_jones, jones, n * 8 * sizeof(float), hipMemcpyHostToDevice));
@@ -762,6 +765,7 @@ int ObitHIPBeamDFTSetBeams(ObitHIPBeam *h, const ObitHIPBeamImageDesc *desc, con
                            const float *beamFreq, float refSizeFact, float fblank) {
   using namespace obit;
   if (!h || !desc || !images || !beamFreq) return fail(kInvalid, "ObitHIPBeamDFTSetBeams: null argument");
+  OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
   const int nt = h->desc.ntype, nch = h->desc.njchan;
   std::vector<beam::BeamImg> bi(nt);
   long long off = 0;
@@ -812,6 +816,7 @@ int ObitHIPBeamDFTJonesFromBeams(ObitHIPBeam *h, const float *pos, float parAng)
   if (!h->d_images) return fail(kState, "ObitHIPBeamDFTJonesFromBeams: no beam images (ObitHIPBeamDFTSetBeams)");
   const int n = h->desc.ncomp;
   if (n == 0) return kOK;
+  OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
   if (!h->d_cpos) OBIT_HIP_TRY(hipMalloc(&h->d_cpos, (size_t)n * 2 * sizeof(float)));
   OBIT_HIP_TRY(hipMemcpy(h->d_cpos, pos, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(beam::k_beam_jones, dim3((unsigned)((n + 255) / 256), (unsigned)h->desc.ntype), dim3(256), 0,
