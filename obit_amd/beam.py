@@ -114,14 +114,21 @@ class BeamDFT:
         """descs: per type dicts (nx, ny, proj, hwidth, crval, crpix, cdelt);
         images float32 [ntype, 8, njchan, ny, nx] (or a list of per-type
         arrays [8, njchan, ny, nx]); beam_freq [njchan] Hz."""
+        imgs = [np.asarray(x, np.float32) for x in images]
+        if len(imgs) != len(descs) or len(descs) != self.desc.ntype:
+            raise ValueError("one image set and one descriptor per antenna type")
         arr = (_lib.BeamImageDesc * len(descs))()
-        for a, dd in zip(arr, descs):
-            a.nx, a.ny, a.nchan = dd["nx"], dd["ny"], self.desc.njchan
+        for a, dd, im in zip(arr, descs, imgs):
+            if im.ndim != 4 or im.shape[0] != 8 or im.shape[2:] != (dd["ny"], dd["nx"]):
+                raise ValueError(f"beam images must be [8, nchan, {dd['ny']}, {dd['nx']}], not {im.shape}")
+            a.nx, a.ny, a.nchan = dd["nx"], dd["ny"], im.shape[1]
             a.proj, a.hwidth = dd.get("proj", 0), dd.get("hwidth", 2)
             for i in range(2):
                 a.crval[i], a.crpix[i], a.cdelt[i] = dd["crval"][i], dd["crpix"][i], dd["cdelt"][i]
-        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).ravel() for x in images]))
+        flat = np.ascontiguousarray(np.concatenate([x.ravel() for x in imgs]))
         bf = np.ascontiguousarray(beam_freq, np.float32)
+        if bf.size != self.desc.njchan:
+            raise ValueError(f"{bf.size} beam frequencies for {self.desc.njchan} beam channels")
         self._beams = (arr, flat, bf)
         _lib.check(self.lib.ObitHIPBeamDFTSetBeams(self.h, arr, flat.ctypes.data, bf.ctypes.data,
                                                    float(ref_size_fact), float(fblank)), "ObitHIPBeamDFTSetBeams")

@@ -291,3 +291,21 @@ def test_beam_antenna_parameters():
         bm.set_jones(jones)
         got = bm.calc(buf2, par_ang=0.5)
     assert np.array_equal(got[:, uv.nrparm + 2:], ref[:, uv.nrparm:])
+
+
+def test_beam_images_refusals():
+    uv = _data(2, rows=64, nif=1, nchan=8)
+    comps = _comps(False, ncomp=4)
+    descs, images = _beams(1, 2, 0, nx=16, ny=16)
+    with BeamDFT(uv, comps, np.zeros(NANT, int), np.arange(8) // 4, 2, ntype=1) as bm:
+        with pytest.raises(Exception, match="no beam images"):
+            bm.jones_from_beams(np.zeros((4, 2), np.float32))
+        with pytest.raises(Exception, match="projection"):
+            bm.set_beams([dict(descs[0], proj=5)], images, [1e9, 1.1e9], 1e-10, FBLANK)
+        with pytest.raises(Exception, match="hwidth"):
+            bm.set_beams([dict(descs[0], hwidth=5)], images, [1e9, 1.1e9], 1e-10, FBLANK)
+    with BeamDFT(uv, comps, np.zeros(NANT, int), np.arange(8) // 2, 4, ntype=1) as bm:
+        with pytest.raises(Exception, match="beam planes"):         # the library's own check
+            bm.set_beams(descs, images, [1e9, 1.1e9, 1.2e9, 1.3e9], 1e-10, FBLANK)
+        with pytest.raises(ValueError, match="beam frequencies"):
+            bm.set_beams(descs, images, [1e9, 1.1e9], 1e-10, FBLANK)
