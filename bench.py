@@ -319,6 +319,63 @@ def bench_beam(lib, uv, sky, reps=2, check_rows=16, check=True):
     return res
 
 
+def bench_beam4(lib, uv, sky, rows=226800, reps=2):
+    """The beam-corrected DFT with 4 correlations (XX, YY, XY, YX), Stokes Q,
+    over the first quarter of workload 2's rows (the cross hands copied
+    from the parallel ones), otherwise as bench_beam; 40 FLOP per
+    evaluation (phasor step 6, amplitude 2, four complex multiply-adds 32)."""
+    from obit_amd import _lib, synth
+    from obit_amd.beam import BeamDFT
+    nf = uv.nif * uv.nchan
+    src = uv.buffer[:rows]
+    d2 = src[:, uv.nrparm:].reshape(rows, nf, 2, 3)
+    buf = np.empty((rows, uv.nrparm + nf * 12), np.float32)
+    buf[:, :uv.nrparm] = src[:, :uv.nrparm]
+    d4 = buf[:, uv.nrparm:].reshape(rows, nf, 4, 3)
+    d4[:, :, :2] = d2
+    d4[:, :, 2:] = d2
+    uv4 = synth.UVData(buffer=buf, nchan=uv.nchan, nif=uv.nif, nstok=4, freq=uv.freq, freqArr=uv.freqArr,
+                       crval_stokes=-5.0, nrparm=uv.nrparm)
+    cb = src[:, uv.ilocb].astype(np.int64)
+    nant = int(max((cb // 256).max(), (cb % 256).max()))
+    njchan = 8
+    jmap = (np.arange(nf) * njchan) // nf
+    comps = np.ascontiguousarray(sky.comps[:, :4], np.float32)
+    rng = np.random.default_rng(11)
+    jones = np.zeros((1, comps.shape[0], njchan, 2, 2), np.complex64)
+    jones[..., 0, 0] = jones[..., 1, 1] = 1.0
+    jones += (0.1 * (rng.normal(size=jones.shape) + 1j * rng.normal(size=jones.shape))).astype(np.complex64)
+    evals = rows * nf * comps.shape[0]
+    d = C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(d), buf.nbytes), "malloc vis")
+    try:
+        _lib.check(lib.ObitHIPMemcpyH2D(d, _lib.ptr(buf), buf.nbytes), "upload vis")
+        with BeamDFT(uv4, comps, np.zeros(nant, np.int32), jmap, njchan, ntype=1, stok_type=2, op="sub") as bm:
+            bm.set_jones(jones)
+            bm.calc_device(d.value, rows, 0.3)
+            lib.ObitHIPDeviceSynchronize()
+            e0, e1 = _events(lib)
+            lib.ObitCUDAEventRecord(e0, None)
+            for _ in range(reps):
+                bm.calc_device(d.value, rows, 0.3)
+            lib.ObitCUDAEventRecord(e1, None)
+            lib.ObitHIPDeviceSynchronize()
+            ms = lib.ObitHIPEventElapsedMs(e0, e1) / reps
+    finally:
+        lib.ObitHIPFree(d)
+    flop = 40.0
+    res = {"workload": f"first {rows} rows of workload 2 x 256 ch, XX/YY/XY/YX, 10,000 points, 1 antenna type, "
+                       "8 beam channels, Stokes Q, subtract",
+           "rows": rows, "channels": nf, "components": int(comps.shape[0]), "ms": ms,
+           "evals_per_s": evals / (ms * 1e-3),
+           "roofline": {"bound": "valu", "achieved": evals * flop / (ms * 1e-3) / 1e12, "peak": 157.3,
+                        "unit": "TFLOP/s", "frac": evals * flop / (ms * 1e-3) / 1e12 / 157.3,
+                        "flop_per_eval": flop}}
+    log(f"beam DFT, 4 correlations: {evals:.3e} evaluations in {ms:.1f} ms = {res['evals_per_s']:.3e}/s "
+        f"({res['roofline']['frac'] * 100:.1f} % of FP32 at {flop:.0f} FLOP each)")
+    return res
+
+
 def bench_stokes(lib, uv, reps=5, check_rows=2000, check=True):
     """Stokes I records of workload 2's resident RR/LL rows
     (ObitHIPStokesI, the gridder's input), timed with HIP events on the
@@ -686,7 +743,8 @@ def main():
     if args.beam_only:
         from obit_amd import synth
         uv, sky, _ = synth.workload(2)
-        print(json.dumps({"beam_dft": bench_beam(lib, uv, sky, check=not args.no_cpu)}), flush=True)
+        print(json.dumps({"beam_dft": bench_beam(lib, uv, sky, check=not args.no_cpu),
+                          "beam_dft_4corr": bench_beam4(lib, uv, sky)}), flush=True)
         return
     if args.grid_only:
         grid, _ = bench_grid(lib, args.grid_steps)

@@ -17,7 +17,8 @@
 // shears) and FP32 tile sums folded into FP64.
 //
 // Work decomposition: a workgroup is 256 consecutive rows x one (IF,
-// 8-channel chunk); each thread owns one row.  When the chunk's channels
+// chunk of 8 channels with 2 correlations, 4 with 4); each thread owns one
+// row.  When the chunk's channels
 // share one beam channel (the usual case: beam channels are coarse), the
 // tile's products for every antenna-type pair are staged in LDS and each
 // thread reads its pair's once per component; otherwise each channel
@@ -35,9 +36,10 @@ namespace beam {
 
 constexpr int kRows = 256;     // rows (threads) per workgroup
 // Channels per thread.  16 with 2 correlations needs 310 VGPRs (one wave
-// per SIMD) and ran 1.54x slower than 8 (two waves per SIMD).
+// per SIMD) and ran 1.54x slower than 8 (two waves per SIMD); with 4
+// correlations 8 needs 314 VGPRs and ran 1.44x slower than 4 (173 VGPRs).
 constexpr int kCPT2 = 8;       // 2 correlations
-constexpr int kCPT4 = 8;       // 4 correlations
+constexpr int kCPT4 = 4;       // 4 correlations
 constexpr int kTile = 64;      // components per LDS tile
 constexpr int kMaxPairs = 16;  // type pairs staged in LDS (4 antenna types)
 constexpr int kMaxTypes = 16;
