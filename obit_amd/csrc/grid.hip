@@ -27,6 +27,8 @@
 #include "host.h"
 
 #include <hipfft/hipfft.h>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 #include "ObitCUDAUtil.h"
 #include "ObitHIPVis.h"
 
@@ -659,6 +661,11 @@ struct GpuState {
   long long listCap = 0;
   int2 *d_items = nullptr;
   long long itemCap = 0;
+  long long *d_itemOff = nullptr;  // first work item of every list (exclusive scan of chunk counts)
+  void *d_scanTmp = nullptr;       // scan scratch
+  size_t scanTmpBytes = 0;
+  long long *h_tot = nullptr;      // pinned: list entries and work items of the last pass
+  long long nitems = 0;
   unsigned long long *d_walk = nullptr;  // staged samples, base-cell moves of the last pass
   // FFT to images: C2R plan for (nx, ny, nplane), complex scratch, corrections
   hipfftHandle fftPlan = 0;
@@ -669,9 +676,6 @@ struct GpuState {
   size_t corrCap = 0;
   std::vector<float> h_corr;  // xCorr, yCorr of the last (nx, ny) (GridCorrFn)
   int corrNx = 0, corrNy = 0;
-  std::vector<int> h_counts;
-  std::vector<long long> h_offsets;
-  std::vector<int2> h_items;
   // tables
   float *d_gcf = nullptr, *d_freq = nullptr, *d_sig = nullptr, *d_pf = nullptr;
   int *d_pchan = nullptr, *d_pcount = nullptr;
@@ -854,11 +858,34 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   OBIT_HIP_TRY(hipMalloc(&gs->d_counts, sizeof(int) * (gs->nlist + 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_cursor, sizeof(int) * (gs->nlist + 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_offsets, sizeof(long long) * (gs->nlist + 1)));
+  (void)hipFree(gs->d_itemOff);
+  OBIT_HIP_TRY(hipMalloc(&gs->d_itemOff, sizeof(long long) * (gs->nlist + 1)));
+  if (!gs->h_tot) OBIT_HIP_TRY(hipHostMalloc(&gs->h_tot, 2 * sizeof(long long), hipHostMallocDefault));
   gs->ready = true;
   return kOK;
 }
 
 // Bin and grid nrow rows at a device pointer into every facet on gs.
+// List sizes as 64-bit entry counts and as kChunk-track work-item counts.
+struct ToEntries {
+  __host__ __device__ long long operator()(int n) const { return (long long)n; }
+};
+struct ToChunks {
+  __host__ __device__ long long operator()(int n) const { return (long long)((n + kChunk - 1) / kChunk); }
+};
+
+// Work items (list, chunk) of every list, in list order, at the list's
+// scanned item offset.
+__global__ __launch_bounds__(256) void k_items(const int *counts, const long long *itemOff, long long nl,
+                                               int2 *items) {
+  for (long long L = (long long)blockIdx.x * blockDim.x + threadIdx.x; L < nl;
+       L += (long long)gridDim.x * blockDim.x) {
+    const int n = counts[L];
+    const long long base = itemOff[L];
+    for (int c = 0; c * kChunk < n; ++c) items[base + c] = make_int2((int)L, c);
+  }
+}
+
 int grid_device(GpuState *gs, const float *vis, long long nrow) {
   if (!gs->ready) return fail(kState, "gridder not initialised");
   if (nrow <= 0 || gs->nlist == 0) return kOK;
@@ -879,24 +906,39 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 0);
   OBIT_HIP_TRY(hipGetLastError());
-  // list sizes to the host: offsets (exclusive scan) and the work items
-  // (list, chunk) that split long lists into kChunk-track pieces
-  gs->h_counts.resize((size_t)nl);
-  OBIT_HIP_TRY(hipMemcpyAsync(gs->h_counts.data(), gs->d_counts, sizeof(int) * nl, hipMemcpyDeviceToHost,
-                              gs->stream));
-  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
-  gs->h_offsets.resize((size_t)nl + 1);
-  gs->h_items.clear();
-  long long total = 0;
-  for (long long L = 0; L < nl; ++L) {
-    gs->h_offsets[L] = total;
-    const int n = gs->h_counts[L];
-    for (int c = 0; c * kChunk < n; ++c) gs->h_items.push_back(make_int2((int)L, c));
-    total += n;
+  // list offsets and work items on the device: exclusive scans of the list
+  // sizes and of their kChunk-track chunk counts (the entry after the last
+  // list is zero, so each scan's last element is its total); only the two
+  // totals come back, to size the lists and the launch
+  {
+    const auto entries = rocprim::make_transform_iterator(gs->d_counts, ToEntries());
+    const auto chunks = rocprim::make_transform_iterator(gs->d_counts, ToChunks());
+    size_t need = 0, need2 = 0;
+    OBIT_HIP_TRY(rocprim::exclusive_scan(nullptr, need, entries, gs->d_offsets, 0LL, (size_t)nl + 1,
+                                         rocprim::plus<long long>(), gs->stream));
+    OBIT_HIP_TRY(rocprim::exclusive_scan(nullptr, need2, chunks, gs->d_itemOff, 0LL, (size_t)nl + 1,
+                                         rocprim::plus<long long>(), gs->stream));
+    need = std::max(need, need2);
+    if (need > gs->scanTmpBytes) {
+      (void)hipFree(gs->d_scanTmp);
+      gs->d_scanTmp = nullptr;
+      OBIT_HIP_TRY(hipMalloc(&gs->d_scanTmp, need));
+      gs->scanTmpBytes = need;
+    }
+    OBIT_HIP_TRY(rocprim::exclusive_scan(gs->d_scanTmp, need, entries, gs->d_offsets, 0LL, (size_t)nl + 1,
+                                         rocprim::plus<long long>(), gs->stream));
+    OBIT_HIP_TRY(rocprim::exclusive_scan(gs->d_scanTmp, need, chunks, gs->d_itemOff, 0LL, (size_t)nl + 1,
+                                         rocprim::plus<long long>(), gs->stream));
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_tot, gs->d_offsets + nl, sizeof(long long), hipMemcpyDeviceToHost,
+                                gs->stream));
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_tot + 1, gs->d_itemOff + nl, sizeof(long long), hipMemcpyDeviceToHost,
+                                gs->stream));
+    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
   }
-  gs->h_offsets[nl] = total;
-  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_offsets, gs->h_offsets.data(), sizeof(long long) * (nl + 1),
-                              hipMemcpyHostToDevice, gs->stream));
+  const long long total = gs->h_tot[0];
+  const long long nitems = gs->h_tot[1];
+  gs->nitems = nitems;
+  if (total < 0 || nitems < 0 || nitems > total) return fail(kHipError, "list scan gave %lld entries, %lld items", total, nitems);
   if ((long long)total > gs->listCap) {
     (void)hipFree(gs->d_lists);
     gs->d_lists = nullptr;
@@ -904,15 +946,18 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
     if (e != hipSuccess) return fail(kNoMemory, "track lists (%lld): %s", total, hipGetErrorString(e));
     gs->listCap = total;
   }
-  const long long nitems = (long long)gs->h_items.size();
   if (nitems > gs->itemCap) {
     (void)hipFree(gs->d_items);
+    gs->d_items = nullptr;
     OBIT_HIP_TRY(hipMalloc(&gs->d_items, sizeof(int2) * (size_t)std::max<long long>(nitems, 1)));
     gs->itemCap = nitems;
   }
-  if (nitems)
-    OBIT_HIP_TRY(hipMemcpyAsync(gs->d_items, gs->h_items.data(), sizeof(int2) * nitems,
-                                hipMemcpyHostToDevice, gs->stream));
+  if (nitems) {
+    const int nbI = (int)std::min<long long>((nl + 255) / 256, (long long)gs->nCU * 16);
+    hipLaunchKernelGGL(k_items, dim3(nbI), dim3(256), 0, gs->stream, gs->d_counts, gs->d_itemOff, nl,
+                       gs->d_items);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
   OBIT_HIP_TRY(hipGetLastError());
@@ -931,7 +976,7 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
                        gs->d_counts, gs->d_offsets, gs->d_lists, gs->d_walk);
     OBIT_HIP_TRY(hipGetLastError());
   }
-  // the host vectors are read by the async copies above
+  // the pass is complete when the call returns
   OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
   return kOK;
 }
@@ -1101,6 +1146,8 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
     (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
     (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis); (void)hipFree(gs->d_walk);
+    (void)hipFree(gs->d_itemOff); (void)hipFree(gs->d_scanTmp);
+    if (gs->h_tot) (void)hipHostFree(gs->h_tot);
     if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
     (void)hipFree(gs->d_fft); (void)hipFree(gs->d_corr);
     if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
@@ -1321,7 +1368,12 @@ int ObitHIPGridListStats(CUDAGridInfo *in, long iGPU, long long *stats) {
   GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
   if (!gs || !stats) return obit::fail(obit::kState, "gridder not allocated");
   long long nonEmpty = 0, entries = 0, maxc = 0, multi = 0;
-  for (int c : gs->h_counts) {
+  std::vector<int> counts((size_t)gs->nlist);
+  if (gs->nlist > 0 && gs->d_counts) {
+    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+    OBIT_HIP_TRY(hipMemcpy(counts.data(), gs->d_counts, sizeof(int) * gs->nlist, hipMemcpyDeviceToHost));
+  }
+  for (int c : counts) {
     if (c > 0) ++nonEmpty;
     entries += c;
     if (c > maxc) maxc = c;
@@ -1330,7 +1382,7 @@ int ObitHIPGridListStats(CUDAGridInfo *in, long iGPU, long long *stats) {
   stats[0] = gs->nlist;
   stats[1] = nonEmpty;
   stats[2] = entries;
-  stats[3] = (long long)gs->h_items.size();
+  stats[3] = gs->nitems;
   stats[4] = maxc;
   stats[5] = multi;
   unsigned long long walk[2] = {0, 0};
