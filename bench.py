@@ -452,6 +452,14 @@ def bench_grid(lib, steps):
             lib.ObitCUDAEventRecord(e1, st)
             lib.ObitHIPDeviceSynchronize()
             ms = lib.ObitHIPEventElapsedMs(e0, e1) / steps
+            # a major cycle's later passes: same uvw, the binning pass reused
+            lib.ObitCUDAEventRecord(e0, st)
+            for _ in range(steps):
+                gg.grid_device_reuse(dptr.value, uv.nrow)
+            lib.ObitCUDAEventRecord(e1, st)
+            lib.ObitHIPDeviceSynchronize()
+            res["ms_per_pass_lists_reused"] = lib.ObitHIPEventElapsedMs(e0, e1) / steps
+            log(f"gridding with the track lists reused: {res['ms_per_pass_lists_reused']:.2f} ms/pass")
             st8 = (C.c_longlong * 8)()
             _lib.check(lib.ObitHIPGridListStats(gg.c, 0, st8), "list stats")
             res["lists"] = dict(zip(("lists", "non_empty", "entries", "items", "longest", "split",

@@ -88,6 +88,13 @@ void *ObitHIPSkyModelDFTStream(GPUInfo *gpuInfo);
 /* Grid nvis resident rows (device pointer, lenvis floats each) into every
  * facet on iGPU, in one binning pass and one tile-gridding launch. */
 int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis);
+/* The same with the track lists of the previous pass on iGPU: for rows
+ * with the same u, v, w (a major cycle regridding residuals), which the
+ * caller guarantees.  The lists depend on uvw and the facet geometry only,
+ * so the binning pass is skipped.  Refused (OBIT_HIP_ESTATE) when the
+ * previous pass had a different row count or the facets were set up
+ * again since. */
+int ObitHIPGridGridDeviceReuse(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis);
 /* Trim the hw guard columns and swap the v halves of every plane of a
  * facet (ObitGPUGrid2Host + ObitCArray2DCenter) on the device:
  * out[nplane][ny][nx/2+1][2]. */

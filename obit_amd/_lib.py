@@ -156,6 +156,7 @@ _PROTOS = {
     "ObitHIPSkyModelDFTSetPrecision": (_I, [_P, _I]),
     "ObitHIPSkyModelDFTStream": (_P, [_P]),
     "ObitHIPGridGridDevice": (_I, [_P, _L, _P, _LL]),
+    "ObitHIPGridGridDeviceReuse": (_I, [_P, _L, _P, _LL]),
     "ObitHIPGrid2Device": (_I, [_P, _L, _P]),
     "ObitHIPGrid2HostPlanes": (_I, [_P, _L, _P]),
     "ObitHIPGridPrepDebug": (_I, [_P, _L, _P, _LL, _P, _P, _P, _P, _P]),

@@ -666,6 +666,8 @@ struct GpuState {
   size_t scanTmpBytes = 0;
   long long *h_tot = nullptr;      // pinned: list entries and work items of the last pass
   long long nitems = 0;
+  bool listsValid = false;         // the lists of the last pass, for ...
+  long long listsRows = 0;         // ... a pass over as many rows with the same uvw
   unsigned long long *d_walk = nullptr;  // staged samples, base-cell moves of the last pass
   // FFT to images: C2R plan for (nx, ny, nplane), complex scratch, corrections
   hipfftHandle fftPlan = 0;
@@ -735,6 +737,7 @@ void build_gcf(const float *convfn, int W, int N, std::vector<float> &gcf) {
 
 // Upload tables and facet parameters from the host descriptors.
 int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
+  gs->listsValid = false;
   GridInfo *gi = in->h_gridInfo;
   if (!gi) return fail(kState, "h_gridInfo not allocated");
   OBIT_HIP_TRY(hipSetDevice(gs->device));
@@ -886,11 +889,23 @@ __global__ __launch_bounds__(256) void k_items(const int *counts, const long lon
   }
 }
 
-int grid_device(GpuState *gs, const float *vis, long long nrow) {
+int launch_taps(GpuState *gs, const float *vis);
+
+int grid_device(GpuState *gs, const float *vis, long long nrow, bool reuse) {
   if (!gs->ready) return fail(kState, "gridder not initialised");
   if (nrow <= 0 || gs->nlist == 0) return kOK;
+  if (reuse && !(gs->listsValid && gs->listsRows == nrow))
+    return fail(kState, "no track lists of a %lld-row pass to reuse", nrow);
   const int nf = (int)gs->facets.size();
   const long long nl = gs->nlist;
+  if (reuse) {
+    // the lists, offsets and work items of the last pass stand: only the
+    // walk statistics restart
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_walk, (int *)gs->d_walk + 2, 2LL);
+    OBIT_HIP_TRY(hipGetLastError());
+    return launch_taps(gs, vis);
+  }
+  gs->listsValid = false;
   if ((long long)nrow * gs->cm.nplane * nf > (1LL << 40)) return fail(kInvalid, "too many tracks");
   {
     const int nbz = (int)std::min<long long>((nl + 1 + 255) / 256, 65535);
@@ -961,6 +976,15 @@ int grid_device(GpuState *gs, const float *vis, long long nrow) {
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
   OBIT_HIP_TRY(hipGetLastError());
+  gs->listsValid = true;
+  gs->listsRows = nrow;
+  return launch_taps(gs, vis);
+}
+
+// The tile-gridding launch over the current work items; the pass is
+// complete when it returns.
+int launch_taps(GpuState *gs, const float *vis) {
+  const long long nitems = gs->nitems;
   if (nitems) {
     // persistent grid of exactly the resident blocks: more would queue a
     // second round of blocks behind the first and idle CUs at the end
@@ -1093,7 +1117,7 @@ void ObitCUDAGridGrid(CUDAGridInfo *cudaInfo, long iGPU, int init) {
     gs->visCap = bytes;
   }
   OBIT_HIP_CHECK(hipMemcpyAsync(gs->d_vis, gi->h_data_in, bytes, hipMemcpyHostToDevice, gs->stream));
-  if (obit::grid::grid_device(gs, gs->d_vis, nvis)) grid_abort("ObitCUDAGridGrid");
+  if (obit::grid::grid_device(gs, gs->d_vis, nvis, false)) grid_abort("ObitCUDAGridGrid");
   OBIT_HIP_CHECK(hipStreamSynchronize(gs->stream));
 }
 
@@ -1195,7 +1219,14 @@ int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long 
     int rc = obit::grid::refresh(in, iGPU, gs);
     if (rc) return rc;
   }
-  return obit::grid::grid_device(gs, d_vis, nvis);
+  return obit::grid::grid_device(gs, d_vis, nvis, false);
+}
+
+int ObitHIPGridGridDeviceReuse(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis) {
+  GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
+  if (!gs) return obit::fail(obit::kState, "gridder not allocated on GPU index %ld", iGPU);
+  OBIT_HIP_TRY(hipSetDevice(gs->device));
+  return obit::grid::grid_device(gs, d_vis, nvis, true);
 }
 
 int ObitHIPGrid2Device(CUDAGridInfo *in, long ifacet, float *d_out) {

@@ -84,6 +84,11 @@ class GPUGrid:
         """Grid nrow rows resident in device memory in one pass."""
         _lib.check(self.lib.ObitHIPGridGridDevice(self.c, igpu, C.c_void_p(dptr), nrow), "GridDevice")
 
+    def grid_device_reuse(self, dptr: int, nrow: int, igpu: int = 0) -> None:
+        """Grid rows with the same u, v, w as the last pass, reusing its
+        track lists (ObitHIPGridGridDeviceReuse)."""
+        _lib.check(self.lib.ObitHIPGridGridDeviceReuse(self.c, igpu, C.c_void_p(dptr), nrow), "GridDeviceReuse")
+
     def flip(self) -> None:
         _lib.check(self.lib.ObitHostGridFlip(self.c), "Flip")
 

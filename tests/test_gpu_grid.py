@@ -207,3 +207,45 @@ def test_grid_split_lists(oracle):
     err = np.abs(g - ref).max() / np.abs(ref).max()
     print(f"split lists: max |dG|/peak {err:.3e}", flush=True)
     assert err < 2e-6
+
+
+def test_grid_reuse_lists(uv4):
+    """A second pass over rows with the same uvw and new visibilities that
+    reuses the first pass's track lists (ObitHIPGridGridDeviceReuse) adds
+    what a full pass adds; a reuse without lists, or over a different row
+    count, is refused."""
+    import ctypes as C
+    from obit_amd import _lib
+    lib = _lib.load()
+    f = _facets_cfg4(2)
+    rng = np.random.default_rng(3)
+    b2 = uv4.buffer.copy()
+    d = b2[:, uv4.nrparm:].reshape(uv4.nrow, -1, 3)
+    d[..., 0] = rng.normal(size=d.shape[:-1])
+    d[..., 1] = rng.normal(size=d.shape[:-1])
+    nbytes = uv4.buffer.nbytes
+    dA, dB = C.c_void_p(), C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(dA), nbytes))
+    _lib.check(lib.ObitHIPMalloc(C.byref(dB), nbytes))
+    try:
+        _lib.check(lib.ObitHIPMemcpyH2D(dA, _lib.ptr(uv4.buffer), nbytes))
+        _lib.check(lib.ObitHIPMemcpyH2D(dB, _lib.ptr(b2), nbytes))
+        with _gpu_grid(uv4, f) as gg:
+            gg.grid_device(dA.value, uv4.nrow)
+            gg.grid_device(dB.value, uv4.nrow)
+            gg.flip()
+            full = [gg.to_host(i) for i in range(2)]
+        with _gpu_grid(uv4, f) as gg:
+            with pytest.raises(Exception, match="reuse"):
+                gg.grid_device_reuse(dB.value, uv4.nrow)
+            gg.grid_device(dA.value, uv4.nrow)
+            gg.grid_device_reuse(dB.value, uv4.nrow)
+            with pytest.raises(Exception, match="reuse"):
+                gg.grid_device_reuse(dB.value, uv4.nrow - 1)
+            gg.flip()
+            reused = [gg.to_host(i) for i in range(2)]
+    finally:
+        lib.ObitHIPFree(dA)
+        lib.ObitHIPFree(dB)
+    for x, y in zip(full, reused):
+        assert np.abs(x - y).max() <= 1e-6 * np.abs(x).max()
