@@ -682,6 +682,12 @@ def bench_workload5(lib, sample_rows=32):
             gms = lib.ObitHIPEventElapsedMs(e0, e1)
             st8 = (C.c_longlong * 8)()
             _lib.check(lib.ObitHIPGridListStats(gg.c, 0, st8), "list stats")
+            # the next major cycle: same uvw, the track lists reused
+            lib.ObitCUDAEventRecord(e0, st)
+            gg.grid_device_reuse(d_i.value, nrow)
+            lib.ObitCUDAEventRecord(e1, st)
+            lib.ObitHIPDeviceSynchronize()
+            res["grid_ms_lists_reused"] = lib.ObitHIPEventElapsedMs(e0, e1)
     finally:
         if d_vis.value:
             lib.ObitHIPFree(d_vis)
@@ -691,7 +697,8 @@ def bench_workload5(lib, sample_rows=32):
     res["grid_samples_per_s"] = samples / (gms * 1e-3)
     res["grid_lists"] = dict(zip(("lists", "non_empty", "entries", "items", "longest", "split",
                                   "tile_samples", "cell_moves"), list(st8)))
-    log(f"workload 5: Stokes I {res['stokes_i_ms']:.1f} ms; regrid {samples:.3e} samples in {gms:.0f} ms")
+    log(f"workload 5: Stokes I {res['stokes_i_ms']:.1f} ms; regrid {samples:.3e} samples in {gms:.0f} ms "
+        f"({res['grid_ms_lists_reused']:.0f} ms with the track lists reused)")
     # the checker: the sampled rows' model (before - after) against the
     # oracle's FP64 DFT, and their Stokes I records gridded on the device
     # against the oracle's FP64 gridder
