@@ -69,6 +69,9 @@ typedef struct {
   int opType;                /* ObitHIPBeamOp* */
   float stokFact[4];         /* factor per correlation XX, XY, YX, YY
                                 (the reference's modelInfo->stokFact) */
+  int chanb, chane;          /* channels (0-rel, inclusive) the model is applied to,
+                                and IFs; outside them the data are left as they are */
+  int IFb, IFe;              /* (visInfo->chanb/chane/IFb/IFe, ObitGPUBeamModel.c:324-333) */
 } ObitHIPBeamDesc;
 
 typedef struct ObitHIPBeam ObitHIPBeam;

@@ -39,7 +39,8 @@ class BeamDesc(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "lrec", "nrparm", "ilocu", "ilocv", "ilocw", "ilocb", "iloca1", "iloca2",
         "nif", "nchan", "incf", "incif", "ncorr", "ncomp", "compLen", "ntype", "nant",
-        "njchan", "stokType", "opType")] + [("stokFact", C.c_float * 4)]
+        "njchan", "stokType", "opType")] + [("stokFact", C.c_float * 4)] + [
+        (n, C.c_int) for n in ("chanb", "chane", "IFb", "IFe")]
 
 
 class BeamImageDesc(C.Structure):

@@ -57,7 +57,8 @@ class BeamDFT:
     """One data layout, component table and Jones table on the device."""
 
     def __init__(self, uv, comps, ant_type, vis_jchan, njchan, ntype=None, stok_type=1, op="sub",
-                 stok_fact=(1.0, 1.0, 1.0, 1.0), ncorr=None, stream=None, iloca=None):
+                 stok_fact=(1.0, 1.0, 1.0, 1.0), ncorr=None, stream=None, iloca=None,
+                 chan_sel=None, if_sel=None):
         self.lib = _lib.load()
         self.uv = uv
         comps = np.ascontiguousarray(comps, np.float32)
@@ -77,6 +78,9 @@ class BeamDFT:
         d.stokType, d.opType = int(stok_type), OPS[op]
         for i in range(4):
             d.stokFact[i] = float(stok_fact[i])
+        # channel / IF selection (0-rel, inclusive); all by default
+        d.chanb, d.chane = chan_sel if chan_sel is not None else (0, uv.nchan - 1)
+        d.IFb, d.IFe = if_sel if if_sel is not None else (0, uv.nif - 1)
         self.desc = d
         f0, df = freq_lines(uv)
         self.h = C.c_void_p()

@@ -61,6 +61,7 @@ struct Desc {
   const float *tab;            // per-channel amplitudes [comp][nif * nchan], or null
   int op;
   float sf[4];                 // XX, XY, YX, YY
+  int chanb, chane, IFb, IFe;  // the selection the model is applied to
 };
 
 // One product per (pair, component, beam channel): beam1 * P * beam2^H,
@@ -170,6 +171,8 @@ __global__ __launch_bounds__(kRows, kBlocksPerCU) void k_beam_dft(Desc d) {
     const int ic = (int)(item / d.nrowblk);
     const int iIF = ic / d.nchunk;
     const int c0 = (ic - iIF * d.nchunk) * kCPT;
+    // items wholly outside the selection (the same for the whole block)
+    if (iIF < d.IFb || iIF > d.IFe || c0 + kCPT - 1 < d.chanb || c0 > d.chane) continue;
     const long long row = (long long)rb * kRows + tid;
     const bool valid = row < d.nrow;
     const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
@@ -341,6 +344,7 @@ __global__ __launch_bounds__(kRows, kBlocksPerCU) void k_beam_dft(Desc d) {
 #pragma unroll
     for (int j = 0; j < kCPT; ++j) {
       if (c0 + j >= d.nchan) break;
+      if (c0 + j < d.chanb || c0 + j > d.chane) continue;
       float *p = base + (long long)(c0 + j) * d.incf;
       const bool repl = d.op == ObitHIPBeamOpRepl;
       bool allBad = (p[2] <= 0.0f) && (p[5] <= 0.0f);
@@ -607,6 +611,9 @@ int ObitHIPBeamDFTCreate(ObitHIPBeam **out, const ObitHIPBeamDesc *desc, const d
     return fail(kInvalid, "ObitHIPBeamDFTCreate: bad antenna types / beam channels");
   if (d.stokType < 1 || d.stokType > 4) return fail(kInvalid, "ObitHIPBeamDFTCreate: stokType must be 1..4");
   if (d.opType < 0 || d.opType > 2) return fail(kInvalid, "ObitHIPBeamDFTCreate: bad opType");
+  if (d.chanb < 0 || d.chane >= d.nchan || d.chanb > d.chane || d.IFb < 0 || d.IFe >= d.nif || d.IFb > d.IFe)
+    return fail(kInvalid, "ObitHIPBeamDFTCreate: channel %d..%d / IF %d..%d selection outside %d x %d", d.chanb,
+                d.chane, d.IFb, d.IFe, d.nchan, d.nif);
   for (int i = 0; i < d.nant; ++i)
     if (antType[i] < 0 || antType[i] >= d.ntype)
       return fail(kInvalid, "ObitHIPBeamDFTCreate: antenna %d has type %d of %d", i, antType[i], d.ntype);
@@ -725,6 +732,7 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   d.tab = h->d_tab;
   d.op = D.opType;
   for (int i = 0; i < 4; ++i) d.sf[i] = D.stokFact[i];
+  d.chanb = D.chanb; d.chane = D.chane; d.IFb = D.IFb; d.IFe = D.IFe;
   long long grid = (long long)std::max(h->ncu, 1) * beam::kBlocksPerCU;
   if (grid > d.nitems) grid = d.nitems;
   void *args[] = {&d};

@@ -471,19 +471,23 @@ def beam_model_fp64(buffer, comps, freq_scale, ant_type, vis_jchan, jones, stok_
     return out * sf[None, None], live
 
 
-def beam_apply(buffer, model, live, nrparm, nif, nchan, incf, incif, ncorr, op="sub"):
+def beam_apply(buffer, model, live, nrparm, nif, nchan, incf, incif, ncorr, op="sub", chan_sel=None,
+               if_sel=None):
     """Apply a [nrow, nif*nchan, 2, 2] model to a copy of the rows as
     Obit's CPU beam DFT does (src/ObitSkyModelVMBeamMF.c:3711-3800), in
     FP32: channels with every correlation flagged are skipped unless
     replacing; each correlation changes where its weight is positive (or
     always when replacing); divide acts on XX and YY only; cross hands only
-    with 4 correlations."""
+    with 4 correlations; only channels and IFs inside the selection
+    (reference ObitCUDABeamModel.cu:180-184)."""
     buf = np.array(buffer, np.float32, copy=True)
+    cb, ce = chan_sel if chan_sel is not None else (0, nchan - 1)
+    ib, ie = if_sel if if_sel is not None else (0, nif - 1)
     m = model.astype(np.complex64)
     repl = op == "repl"
     hands = [(0, 0, 0), (1, 1, 3)] + ([(0, 1, 6), (1, 0, 9)] if ncorr == 4 and op != "div" else [])
-    for i in range(nif):
-        for c in range(nchan):
+    for i in range(ib, ie + 1):
+        for c in range(cb, ce + 1):
             f = i * nchan + c
             o = nrparm + i * incif + c * incf
             wts = [buf[:, o + 2], buf[:, o + 5]] + ([buf[:, o + 8], buf[:, o + 11]] if ncorr == 4 else [])

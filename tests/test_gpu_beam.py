@@ -309,3 +309,32 @@ def test_beam_images_refusals():
             bm.set_beams(descs, images, [1e9, 1.1e9, 1.2e9, 1.3e9], 1e-10, FBLANK)
         with pytest.raises(ValueError, match="beam frequencies"):
             bm.set_beams(descs, images, [1e9, 1.1e9], 1e-10, FBLANK)
+
+
+def test_beam_channel_if_selection():
+    """Only channels chanb..chane of IFs IFb..IFe change (the reference's
+    selection test); the rest is bit-identical."""
+    uv = _data(4, rows=700, nif=2, nchan=40, seed=51)
+    comps = _comps(True)
+    rng = np.random.default_rng(5)
+    at = rng.integers(0, 2, NANT)
+    nf = uv.nif * uv.nchan
+    jmap = (np.arange(nf) // uv.nchan) * 5 + (np.arange(nf) % uv.nchan) // 8
+    njchan = int(jmap.max()) + 1
+    jones = _jones(2, len(comps), njchan, seed=7)
+    sel = dict(chan_sel=(5, 30), if_sel=(1, 1))
+    with BeamDFT(uv, comps, at, jmap, njchan, ntype=2, stok_type=4, op="sub", **sel) as bm:
+        bm.set_jones(jones)
+        got = bm.calc(uv.buffer, par_ang=0.1)
+    model, live = orc.beam_model_fp64(uv.buffer, comps, uv.freqArr / uv.freq, at, jmap, jones, 4, 0.1,
+                                      ilocb=uv.ilocb)
+    want = orc.beam_apply(uv.buffer, model, live, uv.nrparm, uv.nif, uv.nchan, uv.incf, uv.incif, 4, "sub", **sel)
+    moved = np.isnan(orc.beam_apply(uv.buffer, np.full_like(model, complex(np.nan, np.nan)), live, uv.nrparm,
+                                    uv.nif, uv.nchan, uv.incf, uv.incif, 4, "sub", **sel))
+    assert moved.any()
+    d = moved[:, uv.nrparm:].reshape(uv.nrow, uv.nif, uv.nchan, -1)
+    assert not d[:, 0].any() and not d[:, 1, :5].any() and not d[:, 1, 31:].any()
+    assert np.array_equal(got[~moved], uv.buffer[~moved])
+    assert np.abs(got[moved].astype(np.float64) - want[moved]).max() < 1e-5 * np.abs(model[live]).max()
+    with pytest.raises(Exception, match="selection"):
+        BeamDFT(uv, comps, at, jmap, njchan, ntype=2, chan_sel=(10, 40))
