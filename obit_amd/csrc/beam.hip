@@ -645,9 +645,10 @@ int ObitHIPBeamDFTCreate(ObitHIPBeam **out, const ObitHIPBeamDesc *desc, const d
   if (!rc) rc = up((void **)&h->d_g, nullptr, 3 * nc * sizeof(float));
   if (!rc) rc = up((void **)&h->d_jones, nullptr, (size_t)d.ntype * nc * d.njchan * 8 * sizeof(float));
   if (!rc) rc = up((void **)&h->d_coh, nullptr, (size_t)d.ntype * d.ntype * nc * d.njchan * 4 * sizeof(float2));
+  if (!rc) rc = ObitHIPBeamDFTSetJones(h, nullptr);
   if (rc) { release(h); return rc; }
   *out = h;
-  return ObitHIPBeamDFTSetJones(h, nullptr);
+  return kOK;
 }
 
 int ObitHIPBeamDFTSetMod(ObitHIPBeam *h, const float *comps) {
