@@ -292,7 +292,7 @@ __global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, 
 // ---------------- tile gridding ----------------
 
 constexpr int kW = 7;          // GCF support in cells (checked on the host)
-constexpr int kChunk = 512;    // tracks per work item
+constexpr int kChunk = 128;    // tracks per work item (swept 64..2048 on workload 4: 128 best)
 constexpr int kWaves = 4;      // waves per workgroup, each with a private tile
 // A sample's footprint is accumulated through a 16 (u) x 8 (v) window of
 // cells whose origin is its base cell rounded down to a multiple of 8 in u
