@@ -71,7 +71,10 @@ typedef struct {
                                 (the reference's modelInfo->stokFact) */
   int chanb, chane;          /* channels (0-rel, inclusive) the model is applied to,
                                 and IFs; outside them the data are left as they are */
-  int IFb, IFe;              /* (visInfo->chanb/chane/IFb/IFe, ObitGPUBeamModel.c:324-333) */
+  int IFb, IFe;              /* (visInfo->chanb/chane/IFb/IFe, ObitGPUBeamModel.c:324-333).
+                                chane < 0 / IFe < 0 select through the last channel / IF;
+                                a zero-initialised descriptor selects channel 0 of IF 0
+                                only, so set them (-1 for all) */
 } ObitHIPBeamDesc;
 
 typedef struct ObitHIPBeam ObitHIPBeam;

@@ -50,6 +50,15 @@ int ObitHIPCopyKernel(void *dst, const void *src, size_t bytes, void *stream);
  * phase 2*pi*(ul+vm+wn)*nu/nu0 is formed from these exact values. */
 int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq,
                               double uvRefFreq);
+/* Kernel the last DFT launch used: *path = 0 the split-precision kernel,
+ * 1 the all-FP64 kernel.  Without ObitHIPSkyModelDFTSetFreq the channel
+ * frequencies come from the FP32 freqScale; the split kernel walks each
+ * IF's channels along a line fitted to them, and a table whose values sit
+ * off that line by more than 4e-7 (relative; 1e-12 with FP64 frequencies)
+ * runs on the FP64 kernel instead, with a one-time message on stderr.
+ * freqScale is read again whenever its pointer changes or a table has been
+ * uploaded through ObitCUDAUtilHost2GPU / ObitCUDAUtilHostAny2GPU since. */
+int ObitHIPSkyModelDFTGetPath(GPUInfo *gpuInfo, int *path);
 /* Reference frequencies of the component spectra in FP64 (Hz): one value
  * (parametrised spectra: the image reference frequency) or nSpec values
  * (tabulated spectra: the coarse-bin frequencies).  Without this call they
@@ -86,14 +95,17 @@ void *ObitHIPSkyModelDFTStream(GPUInfo *gpuInfo);
 /* ---- Gridding ---- */
 
 /* Grid nvis resident rows (device pointer, lenvis floats each) into every
- * facet on iGPU, in one binning pass and one tile-gridding launch. */
+ * facet on iGPU, in one binning pass and one tile-gridding launch.  At
+ * most 2^31 - 1 rows a call (OBIT_HIP_EINVAL beyond: the track lists hold
+ * 32-bit row indices); a larger data set is gridded in several calls. */
 int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis);
 /* The same with the track lists of the previous pass on iGPU: for rows
- * with the same u, v, w (a major cycle regridding residuals), which the
- * caller guarantees.  The lists depend on uvw and the facet geometry only,
- * so the binning pass is skipped.  Refused (OBIT_HIP_ESTATE) when the
- * previous pass had a different row count or the facets were set up
- * again since. */
+ * with the same u, v, w (a major cycle regridding residuals).  The lists
+ * depend on uvw and the facet geometry only, so the binning pass is
+ * skipped.  Refused (OBIT_HIP_ESTATE) when the previous pass had a
+ * different row count, the facets were set up again since, or the rows'
+ * u, v, w differ from the previous pass's (an order-independent checksum
+ * of their bit patterns, computed on the device). */
 int ObitHIPGridGridDeviceReuse(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis);
 /* Trim the hw guard columns and swap the v halves of every plane of a
  * facet (ObitGPUGrid2Host + ObitCArray2DCenter) on the device:

@@ -145,6 +145,7 @@ _PROTOS = {
     "ObitHIPCopyKernel": (_I, [_P, _P, C.c_size_t, _P]),
     "ObitHIPSkyModelDFTSetFreq": (_I, [_P, _DP, _I, C.c_double]),
     "ObitHIPSkyModelDFTSetSpecRef": (_I, [_P, _DP, _I]),
+    "ObitHIPSkyModelDFTGetPath": (_I, [_P, C.POINTER(C.c_int)]),
     "ObitHIPSkyModelDFTSetVis": (_I, [_P, _P, _LL]),
     "ObitHIPSkyModelDFTAttachVis": (_I, [_P, _P, _LL]),
     "ObitHIPSkyModelDFTGetVis": (_I, [_P, _P, _LL]),

@@ -593,7 +593,7 @@ int ObitHIPBeamDFTCreate(ObitHIPBeam **out, const ObitHIPBeamDesc *desc, const d
   if (!out || !desc || !fIF || !dfIF || !antType || !visJChan)
     return fail(kInvalid, "ObitHIPBeamDFTCreate: null argument");
   *out = nullptr;
-  const ObitHIPBeamDesc &d = *desc;
+  ObitHIPBeamDesc d = *desc;
   if (d.lrec < 1 || d.nrparm < 0 || d.nif < 1 || d.nchan < 1 || d.incf < 1 || d.incif < 1)
     return fail(kInvalid, "ObitHIPBeamDFTCreate: bad record layout");
   if (d.ncorr != 2 && d.ncorr != 4) return fail(kInvalid, "ObitHIPBeamDFTCreate: ncorr must be 2 or 4");
@@ -611,6 +611,9 @@ int ObitHIPBeamDFTCreate(ObitHIPBeam **out, const ObitHIPBeamDesc *desc, const d
     return fail(kInvalid, "ObitHIPBeamDFTCreate: bad antenna types / beam channels");
   if (d.stokType < 1 || d.stokType > 4) return fail(kInvalid, "ObitHIPBeamDFTCreate: stokType must be 1..4");
   if (d.opType < 0 || d.opType > 2) return fail(kInvalid, "ObitHIPBeamDFTCreate: bad opType");
+  // a negative end selects through the last channel / IF
+  if (d.chane < 0) d.chane = d.nchan - 1;
+  if (d.IFe < 0) d.IFe = d.nif - 1;
   if (d.chanb < 0 || d.chane >= d.nchan || d.chanb > d.chane || d.IFb < 0 || d.IFe >= d.nif || d.IFb > d.IFe)
     return fail(kInvalid, "ObitHIPBeamDFTCreate: channel %d..%d / IF %d..%d selection outside %d x %d", d.chanb,
                 d.chane, d.IFb, d.IFe, d.nchan, d.nif);

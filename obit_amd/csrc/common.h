@@ -13,6 +13,11 @@ enum Status { kOK = 0, kInvalid = 1, kHipError = 2, kNoMemory = 3, kState = 4 };
 void set_last_error(const std::string &msg);
 int fail(int code, const char *fmt, ...);
 
+// Count of host-to-device copies made through ObitCUDAUtilHost2GPU /
+// ObitCUDAUtilHostAny2GPU: a cached host copy of a device table is stale
+// once it changes.
+unsigned long long upload_generation();
+
 }  // namespace obit
 
 // ObitCUDA* entry points abort on a HIP error, as the reference's

@@ -59,6 +59,7 @@ struct Desc {
   int nchunk, nrowblk;
   long long nitems;
   const double *fbase, *fdel;  // per selected IF: nu/nu0 at chanb and per channel
+  const double *fchan;         // nu/nu0 of every selected channel [nIFsel][nchsel]
   int ncomp;
   const double *cx, *cy, *cz;  // turns per reference wavelength
   const float *camp;           // amplitude
@@ -266,7 +267,9 @@ __global__ __launch_bounds__(kRows) void k_dft_fp64(Desc d) {
     const long long row = (long long)rb * kRows + tid;
     const bool valid = row < d.nrow;
     const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
-    const double f = d.fbase[isel] + (double)cs * d.fdel[isel];
+    // the channel's own frequency: this kernel also takes tables that are
+    // not a line within an IF
+    const double f = d.fchan[(size_t)isel * d.nchsel + cs];
     const double u = (double)(valid ? rp[d.ilocu] : 0.0f) * f;
     const double v = (double)(valid ? rp[d.ilocv] : 0.0f) * f;
     const double w = (double)(valid ? rp[d.ilocw] : 0.0f) * f;
@@ -417,7 +420,11 @@ struct State {
   std::vector<double> freq64;
   double uvRef = 0.0;
   std::vector<double> specRef;
-  std::vector<float> fscaleHost;  // cached copy of freqScale
+  std::vector<float> fscaleHost;  // cached copy of freqScale ...
+  const float *fscalePtr = nullptr;        // ... read from this device table
+  unsigned long long fscaleGen = ~0ULL;    // ... after this many table uploads
+  bool fallbackNoted = false;     // the FP64 fallback message was printed
+  int lastPath = 0;               // kernel of the last launch: 0 split, 1 FP64
   double *dfreq = nullptr;        // [2*nIFsel]: fbase, fdel
   double *hfreq = nullptr;        // pinned staging
   int freqCap = 0;
@@ -485,11 +492,16 @@ int frequency_lines(State *st, const GPUVisInfo *vi, std::vector<double> &fb,
       return fail(kInvalid, "frequency table has %zu values, need %d", st->freq64.size(), nfreq);
     for (int i = 0; i < nfreq; ++i) f[i] = st->freq64[i] / st->uvRef;
   } else {
-    if (st->fscaleHost.size() < (size_t)nfreq) {
+    // re-read the table when it is another one, has been uploaded again
+    // through the copy utilities, or is longer than the cached copy
+    const unsigned long long gen = upload_generation();
+    if (st->fscaleHost.size() < (size_t)nfreq || st->fscalePtr != vi->freqScale || st->fscaleGen != gen) {
       if (!vi->freqScale) return fail(kInvalid, "no frequency information (freqScale is NULL)");
       st->fscaleHost.resize(nfreq);
       OBIT_HIP_TRY(hipMemcpy(st->fscaleHost.data(), vi->freqScale, sizeof(float) * nfreq,
                              hipMemcpyDeviceToHost));
+      st->fscalePtr = vi->freqScale;
+      st->fscaleGen = gen;
     }
     for (int i = 0; i < nfreq; ++i) f[i] = st->fscaleHost[i];
   }
@@ -566,6 +578,7 @@ int ensure_spec_table(State *st, const GPUInfo *gi) {
   }
   const size_t need = (size_t)st->ncomp * nIFsel * nchpad;
   if (need > st->tabCap) {
+    destroy_graph(st);  // a captured launch reads the old table
     if (st->tab) OBIT_HIP_TRY(hipFree(st->tab));
     st->tab = nullptr;
     hipError_t e = hipMalloc(&st->tab, need * sizeof(float));
@@ -628,23 +641,28 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
   d.nchunk = (d.nchsel + cpt - 1) / cpt;
   d.nrowblk = (int)((nrow + kRows - 1) / kRows);
   d.nitems = (long long)d.nrowblk * d.nIFsel * d.nchunk;
-  // per-IF frequency lines
-  std::vector<double> fb, fdl;
-  int rc = frequency_lines(st, vi, fb, fdl, nullptr);
+  // per-IF frequency lines, then every selected channel's own value
+  std::vector<double> fb, fdl, fch;
+  int rc = frequency_lines(st, vi, fb, fdl, &fch);
   if (rc) return rc;
-  if (d.nIFsel > st->freqCap) {
+  const int nfq = 2 * d.nIFsel + d.nIFsel * d.nchsel;
+  if (nfq > st->freqCap) {
+    destroy_graph(st);  // a captured launch reads the old table
     if (st->dfreq) OBIT_HIP_TRY(hipFree(st->dfreq));
     if (st->hfreq) OBIT_HIP_TRY(hipHostFree(st->hfreq));
-    OBIT_HIP_TRY(hipMalloc(&st->dfreq, sizeof(double) * 2 * d.nIFsel));
-    OBIT_HIP_TRY(hipHostMalloc(&st->hfreq, sizeof(double) * 2 * d.nIFsel, hipHostMallocDefault));
-    st->freqCap = d.nIFsel;
+    st->dfreq = nullptr;
+    st->hfreq = nullptr;
+    OBIT_HIP_TRY(hipMalloc(&st->dfreq, sizeof(double) * nfq));
+    OBIT_HIP_TRY(hipHostMalloc(&st->hfreq, sizeof(double) * nfq, hipHostMallocDefault));
+    st->freqCap = nfq;
   }
   OBIT_HIP_TRY(hipStreamSynchronize(st->stream));  // staging buffer reuse
   for (int i = 0; i < d.nIFsel; ++i) { st->hfreq[i] = fb[i]; st->hfreq[d.nIFsel + i] = fdl[i]; }
-  OBIT_HIP_TRY(hipMemcpyAsync(st->dfreq, st->hfreq, sizeof(double) * 2 * d.nIFsel,
-                              hipMemcpyHostToDevice, st->stream));
+  std::memcpy(st->hfreq + 2 * d.nIFsel, fch.data(), sizeof(double) * fch.size());
+  OBIT_HIP_TRY(hipMemcpyAsync(st->dfreq, st->hfreq, sizeof(double) * nfq, hipMemcpyHostToDevice, st->stream));
   d.fbase = st->dfreq;
   d.fdel = st->dfreq + d.nIFsel;
+  d.fchan = st->dfreq + 2 * d.nIFsel;
   d.ncomp = st->ncomp;
   d.cx = st->cx; d.cy = st->cy; d.cz = st->cz;
   d.camp = st->camp; d.cg = st->cg;
@@ -692,6 +710,15 @@ int run_device(State *st, const GPUInfo *gi, float *vis, long long nrow, bool fp
   Desc d;
   int rc = make_desc(st, gi, vis, nrow, st->cpt, d);
   if (rc) return rc;
+  if (!fp64 && !st->uniform && !st->fallbackNoted) {
+    // the split kernel walks each IF's channels as a line; a table off a
+    // line goes to the FP64 kernel, which is many times slower
+    std::fprintf(stderr,
+                 "obit_hip: channel frequencies are not uniform within an IF; the DFT runs on the "
+                 "FP64 kernel (ObitHIPSkyModelDFTGetPath reports 1)\n");
+    st->fallbackNoted = true;
+  }
+  st->lastPath = (fp64 || !st->uniform) ? 1 : 0;
   if (fp64 || !st->uniform) return launch_fp64(st, d);
   return launch_split(st, d, st->cpt, st->tile, st->wgPerCU);
 }
@@ -754,6 +781,7 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
   OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
   st->modelReady = true;
   st->tabReady = false;
+  st->fscaleHost.clear();  // a new model may come with a new frequency table
   obit::dft::destroy_graph(st);
 }
 
@@ -808,6 +836,14 @@ int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq
   st->uvRef = uvRefFreq;
   st->tabReady = false;
   obit::dft::destroy_graph(st);
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTGetPath(GPUInfo *gpuInfo, int *path) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTGetPath before DFTInit");
+  if (!path) return obit::fail(obit::kInvalid, "path is NULL");
+  *path = st->lastPath;
   return obit::kOK;
 }
 

@@ -33,6 +33,7 @@
 #include "ObitHIPVis.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -243,6 +244,26 @@ __device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *
 __global__ void k_zero2(int *a, int *b, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) { a[i] = 0; b[i] = 0; }
+}
+
+// Order-independent checksum of the rows' u, v, w bit patterns: the track
+// lists of a pass may be reused only for rows with the same uvw.
+__global__ void k_uvw_hash(const float *vis, long long nrow, int lenvis, unsigned long long *out) {
+  unsigned long long h = 0;
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < nrow;
+       r += (long long)gridDim.x * blockDim.x) {
+    const float *row = vis + r * (long long)lenvis;
+    const unsigned long long a = __float_as_uint(row[0]), b = __float_as_uint(row[1]),
+                             c = __float_as_uint(row[2]);
+    unsigned long long x = (a * 0x9E3779B97F4A7C15ULL) ^ (b * 0xC2B2AE3D27D4EB4FULL) ^
+                           (c * 0x165667B19E3779F9ULL) ^ ((unsigned long long)r * 0xD6E8FEB86659FD93ULL);
+    x ^= x >> 29;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 32;
+    h += x;
+  }
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  if ((threadIdx.x & 63) == 0 && h) atomicAdd(out, h);
 }
 
 // mode 0: count tracks per tile of base cells; mode 1: write row indices
@@ -667,7 +688,9 @@ struct GpuState {
   long long *h_tot = nullptr;      // pinned: list entries and work items of the last pass
   long long nitems = 0;
   bool listsValid = false;         // the lists of the last pass, for ...
-  long long listsRows = 0;         // ... a pass over as many rows with the same uvw
+  long long listsRows = 0;         // ... a pass over as many rows ...
+  unsigned long long listsHash = 0;  // ... with the same uvw (k_uvw_hash)
+  unsigned long long *d_hash = nullptr;
   unsigned long long *d_walk = nullptr;  // staged samples, base-cell moves of the last pass
   // FFT to images: C2R plan for (nx, ny, nplane), complex scratch, corrections
   hipfftHandle fftPlan = 0;
@@ -891,14 +914,34 @@ __global__ __launch_bounds__(256) void k_items(const int *counts, const long lon
 
 int launch_taps(GpuState *gs, const float *vis);
 
+// Checksum of the uvw of nrow rows (k_uvw_hash), read back.
+int uvw_hash(GpuState *gs, const float *vis, long long nrow, unsigned long long &h) {
+  if (!gs->d_hash) OBIT_HIP_TRY(hipMalloc(&gs->d_hash, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_hash, (int *)gs->d_hash + 1, 1LL);
+  const int nb = (int)std::min<long long>((nrow + 255) / 256, (long long)gs->nCU * 4);
+  hipLaunchKernelGGL(k_uvw_hash, dim3(std::max(nb, 1)), dim3(256), 0, gs->stream, vis, nrow, gs->cm.lenvis,
+                     gs->d_hash);
+  OBIT_HIP_TRY(hipGetLastError());
+  OBIT_HIP_TRY(hipMemcpyAsync(&h, gs->d_hash, sizeof(h), hipMemcpyDeviceToHost, gs->stream));
+  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+  return kOK;
+}
+
 int grid_device(GpuState *gs, const float *vis, long long nrow, bool reuse) {
   if (!gs->ready) return fail(kState, "gridder not initialised");
+  if (nrow > (long long)INT_MAX)
+    return fail(kInvalid, "%lld rows: the track lists index at most 2^31 - 1 rows a pass", nrow);
   if (nrow <= 0 || gs->nlist == 0) return kOK;
   if (reuse && !(gs->listsValid && gs->listsRows == nrow))
     return fail(kState, "no track lists of a %lld-row pass to reuse", nrow);
   const int nf = (int)gs->facets.size();
   const long long nl = gs->nlist;
   if (reuse) {
+    unsigned long long h = 0;
+    int rc = uvw_hash(gs, vis, nrow, h);
+    if (rc) return rc;
+    if (h != gs->listsHash)
+      return fail(kState, "cannot reuse the track lists: these rows' u, v, w differ from the last pass's");
     // the lists, offsets and work items of the last pass stand: only the
     // walk statistics restart
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_walk, (int *)gs->d_walk + 2, 2LL);
@@ -976,6 +1019,10 @@ int grid_device(GpuState *gs, const float *vis, long long nrow, bool reuse) {
   hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
                      gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
   OBIT_HIP_TRY(hipGetLastError());
+  {
+    int rc = uvw_hash(gs, vis, nrow, gs->listsHash);
+    if (rc) return rc;
+  }
   gs->listsValid = true;
   gs->listsRows = nrow;
   return launch_taps(gs, vis);
@@ -1169,7 +1216,7 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
     (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
-    (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis); (void)hipFree(gs->d_walk);
+    (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis); (void)hipFree(gs->d_walk); (void)hipFree(gs->d_hash);
     (void)hipFree(gs->d_itemOff); (void)hipFree(gs->d_scanTmp);
     if (gs->h_tot) (void)hipHostFree(gs->h_tot);
     if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
@@ -1212,6 +1259,8 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
 // ---- extensions ----
 
 int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis) {
+  if (nvis > (long long)INT_MAX)
+    return obit::fail(obit::kInvalid, "%lld rows: the track lists index at most 2^31 - 1 rows a pass", nvis);
   GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
   if (!gs) return obit::fail(obit::kState, "gridder not allocated on GPU index %ld", iGPU);
   OBIT_HIP_TRY(hipSetDevice(gs->device));
@@ -1223,6 +1272,8 @@ int ObitHIPGridGridDevice(CUDAGridInfo *in, long iGPU, const float *d_vis, long 
 }
 
 int ObitHIPGridGridDeviceReuse(CUDAGridInfo *in, long iGPU, const float *d_vis, long long nvis) {
+  if (nvis > (long long)INT_MAX)
+    return obit::fail(obit::kInvalid, "%lld rows: the track lists index at most 2^31 - 1 rows a pass", nvis);
   GpuState *gs = obit::grid::get_gpu(in, iGPU, false);
   if (!gs) return obit::fail(obit::kState, "gridder not allocated on GPU index %ld", iGPU);
   OBIT_HIP_TRY(hipSetDevice(gs->device));

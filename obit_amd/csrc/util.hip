@@ -8,6 +8,7 @@
 #include <cstring>
 #include <algorithm>
 #include <mutex>
+#include <atomic>
 
 namespace obit {
 
@@ -29,6 +30,11 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+}  // namespace obit
+
+static std::atomic<unsigned long long> g_uploads{0};
+namespace obit {
+unsigned long long upload_generation() { return g_uploads.load(); }
 }  // namespace obit
 
 extern "C" {
@@ -127,9 +133,11 @@ static void copy_any(void *dst, const void *src, size_t n, hipMemcpyKind kind, i
 
 void ObitCUDAUtilHost2GPU(float *GPU, float *host, int memsize, int *stream) {
   copy_any(GPU, host, (size_t)memsize, hipMemcpyHostToDevice, stream);
+  ++g_uploads;
 }
 void ObitCUDAUtilHostAny2GPU(void *GPU, void *host, int memsize, int *stream) {
   copy_any(GPU, host, (size_t)memsize, hipMemcpyHostToDevice, stream);
+  ++g_uploads;
 }
 void ObitCUDAUtilGPU2Host(float *host, float *GPU, int memsize, int *stream) {
   copy_any(host, GPU, (size_t)memsize, hipMemcpyDeviceToHost, stream);

@@ -156,6 +156,13 @@ class SkyModelDFT:
                    "GetVariant")
         return a.value, b.value, c.value
 
+    def path(self) -> int:
+        """Kernel of the last launch: 0 split precision, 1 the FP64 kernel
+        (taken when the channel frequencies are not uniform within an IF)."""
+        p = C.c_int()
+        _lib.check(self.lib.ObitHIPSkyModelDFTGetPath(self.g, C.byref(p)), "GetPath")
+        return p.value
+
     def set_precision(self, prec: int) -> None:
         _lib.check(self.lib.ObitHIPSkyModelDFTSetPrecision(self.g, prec), "SetPrecision")
 

@@ -269,7 +269,8 @@ def workload(cfg: int, rows: int | None = None, seed: int | None = None, data: b
              ncomp: int | None = None):
     """Build workload cfg (1..5).  rows selects a seeded subset of rows (the
     geometry is that of the full workload); ncomp overrides the component
-    count of single-facet models.  Returns (uv, sky, facets)."""
+    count of single-facet models.  The sky model is drawn from its own
+    seeded stream and does not depend on rows.  Returns (uv, sky, facets)."""
     rng = np.random.default_rng(SEED0 + cfg if seed is None else seed)
     if cfg in (1, 3, 4):
         enu = y_array(rng)
@@ -297,6 +298,9 @@ def workload(cfg: int, rows: int | None = None, seed: int | None = None, data: b
             hdr = _rows_subset(hdr, rows, rng)
             uv = make_uv(hdr, 4, 256, 2, f0, freqArr, rng, data=data)
 
+    # the sky has a stream of its own, so the model of a workload is the
+    # same whatever row subset the visibilities take
+    rng = np.random.default_rng([SEED0 + cfg if seed is None else seed, 1])
     facets = []
     if cfg == 1:
         n = ncomp or 1000

@@ -173,12 +173,14 @@ def dft_obit(uv, sky, buffer=None, nthreads=1, **kw):
     return buf
 
 
-def dft_model_fp64(uv, sky, rows=None, nthreads=8, **kw):
-    """FP64 model visibility [nrow, nif, nchan] (complex128)."""
+def dft_model_fp64(uv, sky, rows=None, nthreads=8, freq_ratio=None, **kw):
+    """FP64 model visibility [nrow, nif, nchan] (complex128).  freq_ratio
+    (nu/nu_uvref per channel, FP64) replaces freqArr/freq for u, v, w; the
+    spectral terms still use freqArr."""
     buf = uv.buffer if rows is None else np.ascontiguousarray(uv.buffer[rows])
     buf = np.ascontiguousarray(buf, dtype=np.float32)
     d, keep = dft_desc(uv, sky, nrow=buf.shape[0], **kw)
-    fr = np.ascontiguousarray(uv.freqArr / uv.freq, np.float64)
+    fr = np.ascontiguousarray(uv.freqArr / uv.freq if freq_ratio is None else freq_ratio, np.float64)
     out = np.zeros((buf.shape[0], uv.nif, uv.nchan, 2), np.float64)
     load().orc_dft_model_fp64(C.byref(d), _p(buf), _p(fr), _p(out), 0, buf.shape[0], nthreads)
     return out[..., 0] + 1j * out[..., 1]

@@ -94,3 +94,23 @@ def test_status_and_message_without_gpu():
     rc = lib.ObitHIPSkyModelDFTCalcResident(fake)
     assert rc == 4
     assert b"resident" in lib.ObitHIPLastError()
+
+
+def test_grid_refuses_more_rows_than_the_lists_index():
+    """The track lists hold 32-bit row indices: a resident pass of 2^31 rows
+    or more is refused before anything touches the device."""
+    from obit_amd import _lib
+    lib = _lib.load()
+    fake = C.c_void_p(12345)
+    for fn in (lib.ObitHIPGridGridDevice, lib.ObitHIPGridGridDeviceReuse):
+        rc = fn(fake, 0, C.c_void_p(0), C.c_longlong(2 ** 31))
+        assert rc == 1
+        assert b"2^31" in lib.ObitHIPLastError()
+
+
+def test_dft_path_query_without_state():
+    from obit_amd import _lib
+    lib = _lib.load()
+    p = C.c_int(-1)
+    assert lib.ObitHIPSkyModelDFTGetPath(C.c_void_p(777), C.byref(p)) == 4
+    assert p.value == -1

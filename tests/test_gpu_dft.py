@@ -217,3 +217,123 @@ def test_full_size_workload2_sampled(oracle):
     w0 = uv.buffer[:, 5:].reshape(uv.nrow, -1, 3)[:, :, 2]
     w1_ = out[:, 5:].reshape(uv.nrow, -1, 3)[:, :, 2]
     assert np.array_equal(w0, w1_)
+
+
+def _gauss_cols(k, rng, frac=0.5):
+    """gp1..gp3 columns (ObitSkyModel.c:1939-1947) for k rows, a fraction
+    frac of them Gaussian (FWHM 2-30 arcsec) and the rest points (zeros)."""
+    maj = rng.uniform(2.0, 30.0, k) / 3600.0
+    mnr = maj * rng.uniform(0.3, 1.0, k)
+    pa = rng.uniform(0.0, 180.0, k)
+    gp = np.stack(synth.gauss_coeffs(maj, mnr, pa), axis=-1)
+    gp[rng.uniform(size=k) >= frac] = 0.0
+    return gp
+
+
+def test_gauss_parametric_spectrum(oracle):
+    """GaussSpec, the type of ObitCUDASkyModel.cu:492-597: rows [a, x, y, z,
+    gp1, gp2, gp3, c1, c2] (coefficients at offset 7), half of them points,
+    on workload 2's 4 IF x 64 channel layout, against the FP64 oracle."""
+    uv, sky0, _ = synth.workload(2, rows=256, ncomp=400)
+    rng = np.random.default_rng(21)
+    k = sky0.comps.shape[0]
+    coef = np.stack([rng.normal(-0.7, 0.3, k), rng.normal(0.0, 0.1, k)], axis=-1)
+    comps = np.concatenate([sky0.comps[:, :4], _gauss_cols(k, rng), coef], axis=1).astype(np.float32)
+    sky = synth.SkyModel(comps=comps, modType=11, nSpecTerm=2, imgRefFreq=float(uv.freqArr[128]))
+    out = _run(uv, sky, op="repl")
+    ref = oracle.dft_model_fp64(uv, sky)
+    err = np.abs(_model_from(uv, out) - ref).max() / np.abs(ref).max()
+    print(f"GaussSpec: max {err:.3e}", flush=True)
+    assert err <= TOL
+    # the spectrum matters at this tolerance: without it the model differs
+    flat = synth.SkyModel(comps=comps[:, :7].copy(), modType=1)
+    assert np.abs(oracle.dft_model_fp64(uv, flat) - ref).max() > 100 * TOL * np.abs(ref).max()
+
+
+def test_gauss_tabulated_spectrum(oracle):
+    """GaussTSpec, the type of ObitCUDASkyModel.cu:608-709: rows [a, x, y, z,
+    gp1, gp2, gp3, alpha, S_1..S_4] with 4 coarse bins, half of them points,
+    against the FP64 oracle."""
+    uv, sky0, _ = synth.workload(2, rows=256, ncomp=300)
+    rng = np.random.default_rng(23)
+    nspec = 4
+    k = sky0.comps.shape[0]
+    alpha = rng.normal(-0.7, 0.3, k)
+    S = np.abs(sky0.comps[:, :1]) * rng.uniform(0.5, 1.5, (k, nspec))
+    comps = np.concatenate([sky0.comps[:, :4], _gauss_cols(k, rng), alpha[:, None], S],
+                           axis=1).astype(np.float32)
+    specFreq = np.array([uv.freqArr[i * 64 + 32] for i in range(4)])
+    specIndex = np.repeat(np.arange(4), 64).astype(np.int32)
+    sky = synth.SkyModel(comps=comps, modType=21, specFreq=specFreq, specIndex=specIndex)
+    for path in ("resident", "buffer"):
+        out = _run(uv, sky, op="repl", path=path)
+        ref = oracle.dft_model_fp64(uv, sky)
+        err = np.abs(_model_from(uv, out) - ref).max() / np.abs(ref).max()
+        print(f"GaussTSpec ({path}): max {err:.3e}", flush=True)
+        assert err <= TOL
+
+
+def _lsq_lines(fs32, nif, nchan):
+    """Per-IF least-squares line through an FP32 freqScale table (FP64)."""
+    out = np.empty(nif * nchan)
+    c = np.arange(nchan, dtype=np.float64)
+    for i in range(nif):
+        y = fs32[i * nchan:(i + 1) * nchan].astype(np.float64)
+        b, a = np.polyfit(c, y, 1)
+        out[i * nchan:(i + 1) * nchan] = a + b * c
+    return out
+
+
+def test_fp32_freqscale_caller(oracle):
+    """The frequencies an unmodified ObitGPUSkyModel caller hands over: the
+    FP32 fscale table only (ObitGPUSkyModel.c:265), no FP64 frequencies,
+    through the per-buffer ObitCUDASkyModelDFTCalc on workload 2's layout.
+    The library fits each IF's channels with a line; the test pins (1) that
+    the line recovers the true frequencies far below FP32 rounding, (2) the
+    kernel at those frequencies to the split tolerance, and (3) that the
+    result is no less accurate against the true sky than Obit's CPU DFT fed
+    the same FP32 table."""
+    from obit_amd.skymodel import SkyModelDFT
+    uv, sky, _ = synth.workload(2, rows=200, ncomp=2000)
+    with SkyModelDFT(uv, sky, op="repl", useFreq64=False, nVisPIO=64) as sm:
+        out = sm.calc(uv.buffer.copy())
+        assert sm.path() == 0  # the split kernel ran
+    got = _model_from(uv, out)
+    true = uv.freqArr / uv.freq
+    fit = _lsq_lines(uv.fscale, uv.nif, uv.nchan)
+    ferr = np.abs(fit / true - 1.0).max()
+    f32err = np.abs(uv.fscale.astype(np.float64) / true - 1.0).max()
+    print(f"frequency: line {ferr:.2e}, FP32 table {f32err:.2e} (relative)", flush=True)
+    assert ferr < 0.5 * f32err
+    ref_fit = oracle.dft_model_fp64(uv, sky, freq_ratio=fit)
+    peak = np.abs(ref_fit).max()
+    e_fit = np.abs(got - ref_fit).max() / peak
+    ref = oracle.dft_model_fp64(uv, sky)
+    cpu = _model_from(uv, oracle.dft_obit(uv, sky, op="repl", nthreads=8))
+    e_gpu = np.abs(got - ref).max() / peak
+    e_cpu = np.abs(cpu - ref).max() / peak
+    print(f"vs FP64 at the fitted frequencies {e_fit:.2e}; vs the true sky: GPU {e_gpu:.2e}, "
+          f"Obit CPU {e_cpu:.2e}", flush=True)
+    assert e_fit <= TOL
+    assert e_gpu <= max(TOL, e_cpu)
+
+
+@pytest.mark.parametrize("freq64", [False, True])
+def test_nonuniform_channels_report_fp64_kernel(oracle, freq64):
+    """A channel 3 kHz off its IF's line: the split kernel cannot walk it,
+    the launch runs on the FP64 kernel (reported by ObitHIPSkyModelDFTGetPath)
+    at every channel's own frequency."""
+    import dataclasses
+    from obit_amd.skymodel import SkyModelDFT
+    uv, sky, _ = synth.workload(2, rows=128, ncomp=500)
+    fa = uv.freqArr.copy()
+    fa[70] += 3.0e3
+    uv = dataclasses.replace(uv, freqArr=fa)
+    with SkyModelDFT(uv, sky, op="repl", useFreq64=freq64, nVisPIO=64) as sm:
+        out = sm.calc(uv.buffer.copy())
+        assert sm.path() == 1
+    fr = uv.freqArr / uv.freq if freq64 else uv.fscale.astype(np.float64)
+    ref = oracle.dft_model_fp64(uv, sky, freq_ratio=fr)
+    err = np.abs(_model_from(uv, out) - ref).max() / np.abs(ref).max()
+    print(f"non-uniform table (freq64={freq64}): max {err:.3e}", flush=True)
+    assert err <= 2e-7  # FP32 output rounding only

@@ -244,8 +244,88 @@ def test_grid_reuse_lists(uv4):
                 gg.grid_device_reuse(dB.value, uv4.nrow - 1)
             gg.flip()
             reused = [gg.to_host(i) for i in range(2)]
+        # as many rows but other u, v, w (another chunk of the data set):
+        # refused, and the grids are left as they were
+        b3 = b2.copy()
+        b3[:, 0] *= np.float32(1.001)
+        _lib.check(lib.ObitHIPMemcpyH2D(dB, _lib.ptr(b3), nbytes))
+        with _gpu_grid(uv4, f) as gg:
+            gg.grid_device(dA.value, uv4.nrow)
+            before = gg.to_host(0)
+            with pytest.raises(Exception, match="differ"):
+                gg.grid_device_reuse(dB.value, uv4.nrow)
+            assert np.array_equal(gg.to_host(0), before)
     finally:
         lib.ObitHIPFree(dA)
         lib.ObitHIPFree(dB)
     for x, y in zip(full, reused):
         assert np.abs(x - y).max() <= 1e-6 * np.abs(x).max()
+
+
+def test_workload4_facets_at_size(oracle):
+    """Workload 4 at its facet size: 8 coarse bins (32 channels each, all
+    channels gridded) of 2048^2 for the centre facet and the outermost
+    facet of the 40-facet hex layout (the largest shift and rotation), on a
+    seeded 2,000-row sample of the full geometry, against the FP64 oracle
+    plane by plane."""
+    from obit_amd.gridder import GPUGrid, plane_of_channel
+    uv, _, facets = synth.workload(4, rows=2000)
+    fs = [facets[0], facets[39]]
+    assert fs[0].nx == 2048 and len(facets) == 40
+    b, e = synth.coarse_bins(256, 8)
+    poc = plane_of_channel(256, b, e, cpu_parity=False)
+    worst = 0.0
+    with GPUGrid(uv, fs, nplane=8, plane_of_chan=poc) as gg:
+        gg.grid(uv.buffer)
+        gg.flip()
+        for i, f in enumerate(fs):
+            g = gg.to_host(i)
+            assert g.shape == (8, 2048, 1025)
+            for p in range(8):
+                ref = oracle.trim_centre(oracle.grid_fp64(uv, f, bChan=int(b[p]), eChan=int(e[p]) + 1), f)
+                err = np.abs(g[p] - ref).max() / np.abs(ref).max()
+                worst = max(worst, err)
+    print(f"2048^2 facets 0 and 39 x 8 planes: max |dG|/peak {worst:.3e}", flush=True)
+    assert worst < 2e-6
+
+
+def test_workload4_full_pass_linearity():
+    """The full workload 4 (210,600 rows x 256 channels into 40 facets x 8
+    planes of 2048^2) in one resident pass equals, cell by cell, the sum of
+    the passes over its even and its odd rows: a size-independent check
+    that no sample is lost or gridded twice at the full list sizes (split
+    lists, halo atomics), on four facets' planes."""
+    import ctypes as C
+    from obit_amd import _lib
+    from obit_amd.gridder import GPUGrid, plane_of_channel
+    lib = _lib.load()
+    uv, _, facets = synth.workload(4)
+    assert uv.nrow == 210600
+    b, e = synth.coarse_bins(256, 8)
+    poc = plane_of_channel(256, b, e, cpu_parity=False)
+    halves = [np.ascontiguousarray(uv.buffer[0::2]), np.ascontiguousarray(uv.buffer[1::2])]
+    check = [0, 7, 20, 39]
+    dptr = C.c_void_p()
+    _lib.check(lib.ObitHIPMalloc(C.byref(dptr), uv.buffer.nbytes))
+    try:
+        out = []
+        for buf in [uv.buffer] + halves:
+            _lib.check(lib.ObitHIPMemcpyH2D(dptr, _lib.ptr(buf), buf.nbytes))
+            with GPUGrid(uv, facets, nplane=8, plane_of_chan=poc) as gg:
+                gg.grid_device(dptr.value, buf.shape[0])
+                st = (C.c_longlong * 8)()
+                _lib.check(lib.ObitHIPGridListStats(gg.c, 0, st))
+                out.append([gg.to_host(i) for i in check])
+            print(f"{buf.shape[0]} rows: entries {st[2]}, items {st[3]}, split lists {st[5]}, "
+                  f"tile samples {st[6]}", flush=True)
+    finally:
+        lib.ObitHIPFree(dptr)
+    worst = 0.0
+    for k in range(len(check)):
+        full, a, bb = out[0][k], out[1][k], out[2][k]
+        for p in range(8):
+            pk = np.abs(full[p]).max()
+            assert pk > 0
+            worst = max(worst, float(np.abs(full[p] - (a[p] + bb[p])).max() / pk))
+    print(f"full pass vs even + odd rows: max |d|/peak {worst:.3e}", flush=True)
+    assert worst < 1e-5
