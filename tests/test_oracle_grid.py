@@ -31,6 +31,83 @@ def test_sphfn_values(oracle):
     assert all(v[i] >= v[i + 1] for i in range(100))
 
 
+def _spheroidal_eigenfunction(c, alpha, n=96):
+    """The even generalised prolate spheroidal function of order 0 from its
+    defining integral equation,
+        lam psi(eta) = int_{-1}^{1} cos(c eta t) (1 - t^2)^alpha psi(t) dt,
+    solved by Gauss-Legendre quadrature (Nystrom): the dominant eigenvector
+    of the symmetrised kernel, interpolated through the equation itself and
+    normalised to psi(0) = 1.  Returns psi(eta) for arrays of eta."""
+    t, w = np.polynomial.legendre.leggauss(n)
+    d = w * (1.0 - t ** 2) ** alpha
+    S = np.sqrt(d)[:, None] * np.cos(c * np.outer(t, t)) * np.sqrt(d)[None, :]
+    ev, V = np.linalg.eigh(S)
+    lam, psi_t = ev[-1], V[:, -1] / np.sqrt(d)
+
+    def psi(eta):
+        eta = np.atleast_1d(np.asarray(eta, np.float64))
+        return (np.cos(c * np.outer(eta, t)) * d[None, :]) @ psi_t / lam
+
+    p0 = psi(0.0)[0]
+    return lambda eta: psi(eta) / p0
+
+
+GCF_TOL = 1e-6  # |table - (1 - eta^2) psi(eta)|; FP32 rounding of the table is ~6e-8
+
+
+def _gcf_reference():
+    """(1 - eta^2) psi(eta) at the table's abscissae eta = i/299, i = 0..299:
+    Schwab's optimal gridding function for support m = 6 and alpha = 1
+    (c = pi m / 2), which ConvFunc(5) tabulates (ObitUVGrid.c:1352-1395)."""
+    eta = np.arange(300) / 299.0
+    return eta, (1.0 - eta ** 2) * _spheroidal_eigenfunction(3.0 * math.pi, 1.0)(eta)
+
+
+def test_gcf_is_the_spheroidal_eigenfunction(oracle):
+    """The library's GCF table (the one the device grids with) and the
+    oracle's equal the gridding function computed from its definition, not
+    from the rational approximation both of them evaluate."""
+    from obit_amd.gridder import conv_func_sph
+    eta, ref = _gcf_reference()
+    h = conv_func_sph()
+    cf, _ = oracle.convfunc()
+    for tab in (h, cf):
+        assert np.abs(tab[350:650] - ref).max() < GCF_TOL
+        assert np.array_equal(tab[51:350], tab[649:350:-1])  # mirrored half
+    # psi itself across both branches of the approximation (eta = 0.75 joins them)
+    lib = oracle.load()
+    e = np.linspace(0.0, 1.0, 401, dtype=np.float32)
+    approx = np.array([lib.orc_sphfn(3, 6, 1, x) for x in e])
+    assert np.abs(approx - _spheroidal_eigenfunction(3.0 * math.pi, 1.0)(e)).max() < GCF_TOL
+    # the check is sharp in c: 0.1 % off pi m / 2 fails it
+    off = (1.0 - eta ** 2) * _spheroidal_eigenfunction(3.003 * math.pi, 1.0)(eta)
+    assert np.abs(h[350:650] - off).max() > 100 * GCF_TOL
+
+
+def test_gcf_check_catches_a_coefficient_typo():
+    """A planted third-significant-digit typo in any coefficient of the
+    upper branch (|eta| > 0.75) of the alpha = 1, m = 6 rational
+    approximation (Schwab's coefficients, ObitUVGrid.c:1471-1490) moves the
+    tabulated function by more than the tolerance of the check above."""
+    P = [4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2]
+    Q = [9.599102e-1, 2.918724e-1]
+    eta, ref = _gcf_reference()
+    up = eta > 0.75
+
+    def table(P, Q):
+        x = eta[up] ** 2 - 1.0
+        num = P[0] + x * (P[1] + x * (P[2] + x * (P[3] + x * P[4])))
+        return (1.0 - eta[up] ** 2) * num / (1.0 + x * (Q[0] + x * Q[1]))
+
+    assert np.abs(table(P, Q) - ref[up]).max() < GCF_TOL  # the clean coefficients pass
+    for coef, k in [(P, k) for k in range(5)] + [(Q, k) for k in range(2)]:
+        for sign in (1.0, -1.0):
+            bad = list(coef)
+            bad[k] += sign * 10.0 ** (math.floor(math.log10(abs(coef[k]))) - 2)
+            t = table(bad, Q) if coef is P else table(P, bad)
+            assert np.abs(t - ref[up]).max() > GCF_TOL, (k, sign)
+
+
 def test_convfn_table(oracle):
     cf, cg = oracle.convfunc()
     assert abs(cf[350] - 1.0) < 1e-6  # normalised peak at the centre
