@@ -510,7 +510,7 @@ def cpu_grid_baseline(seconds):
     t0 = time.perf_counter()
     done = 0
     for f in facets:
-        orc.grid_obit_threaded(uv, f, nGpI=ngpi, nthreads=cores)
+        orc.grid_obit_threaded(uv, f, nGpI=ngpi, nthreads=cores, cpu_build=True)
         done += 1
         if time.perf_counter() - t0 >= seconds:
             break
@@ -519,7 +519,8 @@ def cpu_grid_baseline(seconds):
     log(f"cpu gridding: {done} facets x {uv.nrow} rows x 256 ch in {dt:.1f}s on {cores} threads "
         f"-> {n / dt / 1e9:.3f} Gsamples/s")
     return {"value": n / dt, "unit": "samples/s", "cores": cores, "kind": "port",
-            "sample": f"{done} of the 40 facets, every row and channel ({n:.3g} samples, {dt:.1f} s)"}
+            "sample": f"{done} of the 40 facets, every row and channel ({n:.3g} samples, {dt:.1f} s)",
+            "build": orc.CPU_BUILD_FLAGS}
 
 
 def _cores():
@@ -543,7 +544,7 @@ def cpu_baseline(uv, sky, seconds):
         rows = np.sort(rng.choice(uv.nrow, n, replace=False))
         sub = uv.subset(rows)
         t0 = time.perf_counter()
-        orc.dft_obit(sub, sky, op="sub", nthreads=cores)
+        orc.dft_obit(sub, sky, op="sub", nthreads=cores, cpu_build=True)
         dt = time.perf_counter() - t0
         if dt >= 0.5 * seconds or n >= uv.nrow:
             break
@@ -553,7 +554,8 @@ def cpu_baseline(uv, sky, seconds):
     log(f"cpu baseline: {n} rows in {dt:.1f}s on {cores} threads -> {rate / 1e9:.3f} Gevals/s")
     return {"value": rate, "unit": "evals/s", "cores": cores, "kind": "port",
             "sample": f"{n} seeded rows of the 907,200-row workload, all 256 channels and "
-                      f"10,000 components ({n * per_comp:.3g} evals, {dt:.1f} s)"}
+                      f"10,000 components ({n * per_comp:.3g} evals, {dt:.1f} s)",
+            "build": orc.CPU_BUILD_FLAGS}
 
 
 def copy_calibration(lib, mib):

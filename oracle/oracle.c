@@ -133,6 +133,92 @@ float orc_exp_cephes(float xin)
   return y * p.f;
 }
 
+#ifdef ORC_SSE4LANE
+/* The same two functions four lanes at a time in SSE2 registers, as the
+ * reference's sincos_ps / exp_ps evaluate them (include/sse_mathfun.h:
+ * 214-289, 568-711): the identical operation sequence per lane, so every
+ * lane is bit-identical to orc_sincos_cephes / orc_exp_cephes.  Built only
+ * into the CPU-baseline library (liboracle_cpu.so), which times Obit's
+ * CPU path as a stock "-O3 -msse" build runs it; the checker keeps the
+ * scalar form.  Tested bit for bit against the scalar lanes. */
+#include <emmintrin.h>
+
+static void sincos4(const float *in, float *so, float *co)
+{
+  const __m128i signbit = _mm_set1_epi32((int)0x80000000u);
+  __m128 xin = _mm_loadu_ps(in);
+  __m128 sign_sin = _mm_and_ps(xin, _mm_castsi128_ps(signbit));
+  __m128 x = _mm_andnot_ps(_mm_castsi128_ps(signbit), xin);
+  __m128 y = _mm_mul_ps(x, _mm_set1_ps(C_FOPI));
+  __m128i j = _mm_cvttps_epi32(y);
+  j = _mm_add_epi32(j, _mm_set1_epi32(1));
+  j = _mm_and_si128(j, _mm_set1_epi32(~1));
+  y = _mm_cvtepi32_ps(j);
+  __m128i swap_sin = _mm_slli_epi32(_mm_and_si128(j, _mm_set1_epi32(4)), 29);
+  __m128i poly = _mm_cmpeq_epi32(_mm_and_si128(j, _mm_set1_epi32(2)), _mm_setzero_si128());
+  __m128 t1 = _mm_mul_ps(y, _mm_set1_ps(C_MDP1));
+  __m128 t2 = _mm_mul_ps(y, _mm_set1_ps(C_MDP2));
+  __m128 t3 = _mm_mul_ps(y, _mm_set1_ps(C_MDP3));
+  x = _mm_add_ps(x, t1);
+  x = _mm_add_ps(x, t2);
+  x = _mm_add_ps(x, t3);
+  __m128i sign_cos = _mm_slli_epi32(_mm_andnot_si128(_mm_sub_epi32(j, _mm_set1_epi32(2)),
+                                                     _mm_set1_epi32(4)), 29);
+  sign_sin = _mm_xor_ps(sign_sin, _mm_castsi128_ps(swap_sin));
+  __m128 z = _mm_mul_ps(x, x);
+  __m128 yc = _mm_set1_ps(C_COS0);
+  yc = _mm_mul_ps(yc, z); yc = _mm_add_ps(yc, _mm_set1_ps(C_COS1));
+  yc = _mm_mul_ps(yc, z); yc = _mm_add_ps(yc, _mm_set1_ps(C_COS2));
+  yc = _mm_mul_ps(yc, z); yc = _mm_mul_ps(yc, z);
+  __m128 tmp = _mm_mul_ps(z, _mm_set1_ps(0.5f));
+  yc = _mm_sub_ps(yc, tmp);
+  yc = _mm_add_ps(yc, _mm_set1_ps(1.0f));
+  __m128 ys = _mm_set1_ps(C_SIN0);
+  ys = _mm_mul_ps(ys, z); ys = _mm_add_ps(ys, _mm_set1_ps(C_SIN1));
+  ys = _mm_mul_ps(ys, z); ys = _mm_add_ps(ys, _mm_set1_ps(C_SIN2));
+  ys = _mm_mul_ps(ys, z); ys = _mm_mul_ps(ys, x);
+  ys = _mm_add_ps(ys, x);
+  __m128 pm = _mm_castsi128_ps(poly);
+  __m128 ysin2 = _mm_and_ps(pm, ys);
+  __m128 ysin1 = _mm_andnot_ps(pm, yc);
+  __m128 ys_r = _mm_sub_ps(ys, ysin2);
+  __m128 yc_r = _mm_sub_ps(yc, ysin1);
+  __m128 sn = _mm_add_ps(ysin1, ysin2);
+  __m128 cs = _mm_add_ps(yc_r, ys_r);
+  _mm_storeu_ps(so, _mm_xor_ps(sn, sign_sin));
+  _mm_storeu_ps(co, _mm_xor_ps(cs, _mm_castsi128_ps(sign_cos)));
+}
+
+static void exp4(const float *in, float *out)
+{
+  __m128 x = _mm_loadu_ps(in);
+  x = _mm_min_ps(x, _mm_set1_ps(C_EXPHI));
+  x = _mm_max_ps(x, _mm_set1_ps(C_EXPLO));
+  __m128 fx = _mm_mul_ps(x, _mm_set1_ps(C_LOG2EF));
+  fx = _mm_add_ps(fx, _mm_set1_ps(0.5f));
+  __m128 tmp = _mm_cvtepi32_ps(_mm_cvttps_epi32(fx));
+  __m128 mask = _mm_and_ps(_mm_cmpgt_ps(tmp, fx), _mm_set1_ps(1.0f));
+  fx = _mm_sub_ps(tmp, mask);
+  tmp = _mm_mul_ps(fx, _mm_set1_ps(C_EXPC1));
+  __m128 z = _mm_mul_ps(fx, _mm_set1_ps(C_EXPC2));
+  x = _mm_sub_ps(x, tmp);
+  x = _mm_sub_ps(x, z);
+  z = _mm_mul_ps(x, x);
+  __m128 y = _mm_set1_ps(C_EXPP0);
+  y = _mm_mul_ps(y, x); y = _mm_add_ps(y, _mm_set1_ps(C_EXPP1));
+  y = _mm_mul_ps(y, x); y = _mm_add_ps(y, _mm_set1_ps(C_EXPP2));
+  y = _mm_mul_ps(y, x); y = _mm_add_ps(y, _mm_set1_ps(C_EXPP3));
+  y = _mm_mul_ps(y, x); y = _mm_add_ps(y, _mm_set1_ps(C_EXPP4));
+  y = _mm_mul_ps(y, x); y = _mm_add_ps(y, _mm_set1_ps(C_EXPP5));
+  y = _mm_mul_ps(y, z);
+  y = _mm_add_ps(y, x);
+  y = _mm_add_ps(y, _mm_set1_ps(1.0f));
+  __m128i e = _mm_add_epi32(_mm_cvttps_epi32(fx), _mm_set1_epi32(0x7f));
+  __m128 p2n = _mm_castsi128_ps(_mm_slli_epi32(e, 23));
+  _mm_storeu_ps(out, _mm_mul_ps(y, p2n));
+}
+#endif
+
 /* ObitSinCosVec, stock SSE build (src/ObitSinCos.c:105-186): a single
  * element goes to libm sinf/cosf (include/ObitSinCos.h:59-62); otherwise
  * each lane is sincos_ps (zero-padding of the tail does not change the
@@ -141,14 +227,36 @@ void orc_sincos_vec(int n, const float *ang, float *s, float *c)
 {
   if (n <= 0) return;
   if (n == 1) { s[0] = sinf(ang[0]); c[0] = cosf(ang[0]); return; }
+#ifdef ORC_SSE4LANE
+  int i = 0;
+  for (; i + 4 <= n; i += 4) sincos4(ang + i, s + i, c + i);
+  if (i < n) {
+    float a[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ss[4], cc[4];
+    for (int k = i; k < n; k++) a[k - i] = ang[k];
+    sincos4(a, ss, cc);
+    for (int k = i; k < n; k++) { s[k] = ss[k - i]; c[k] = cc[k - i]; }
+  }
+#else
   for (int i = 0; i < n; i++) orc_sincos_cephes(ang[i], &s[i], &c[i]);
+#endif
 }
 
 /* ObitExpVec, SSE branch (src/ObitExp.c:106-129): every element through
  * exp_ps, including a single one. */
 void orc_exp_vec(int n, const float *arg, float *out)
 {
+#ifdef ORC_SSE4LANE
+  int i = 0;
+  for (; i + 4 <= n; i += 4) exp4(arg + i, out + i);
+  if (i < n) {
+    float a[4] = {0.0f, 0.0f, 0.0f, 0.0f}, o[4];
+    for (int k = i; k < n; k++) a[k - i] = arg[k];
+    exp4(a, o);
+    for (int k = i; k < n; k++) out[k] = o[k - i];
+  }
+#else
   for (int i = 0; i < n; i++) out[i] = orc_exp_cephes(arg[i]);
+#endif
 }
 
 /* ------------------------------------------------------------------ */

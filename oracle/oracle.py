@@ -15,6 +15,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the same source built as Obit's stock "-O3 -msse" build, with the Cephes
+# sincos/exp four lanes at a time: what bench.py times as the CPU baseline
+LIB_CPU = os.path.join(HERE, "liboracle_cpu.so")
+CPU_BUILD_FLAGS = "gcc -O3 -msse -msse2 -mno-fma, Cephes sincos/exp 4 lanes per SSE instruction"
 
 
 def build() -> None:
@@ -59,6 +63,7 @@ class GridDesc(C.Structure):
 
 
 _lib = None
+_lib_cpu = None
 
 
 def load():
@@ -66,7 +71,23 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        lib = C.CDLL(LIB)
+        _lib = _declare(C.CDLL(LIB))
+    return _lib
+
+
+def load_cpu():
+    """The CPU-baseline build (liboracle_cpu.so): same functions, same
+    results bit for bit, built and vectorised as Obit's stock build."""
+    global _lib_cpu
+    if _lib_cpu is None:
+        if not os.path.exists(LIB_CPU):
+            build()
+        _lib_cpu = _declare(C.CDLL(LIB_CPU))
+    return _lib_cpu
+
+
+def _declare(lib):
+    if True:
         P = C.c_void_p
         lib.orc_sincos_cephes.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         lib.orc_exp_cephes.argtypes = [C.c_float]
@@ -86,8 +107,7 @@ def load():
         lib.orc_grid_flip_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int]
         lib.orc_grid_merge.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
         lib.orc_grid_obit_threaded.argtypes = [C.POINTER(GridDesc), P, C.c_int, C.c_int, P]
-        _lib = lib
-    return _lib
+    return lib
 
 
 def _p(a):
@@ -165,11 +185,12 @@ def dft_desc(uv, sky, op="sub", stokFactor=1.0, doFlip=False, startChan=0, numCh
     return d, keep
 
 
-def dft_obit(uv, sky, buffer=None, nthreads=1, **kw):
-    """Obit's CPU DFT (restated) applied in place to a copy of uv.buffer."""
+def dft_obit(uv, sky, buffer=None, nthreads=1, cpu_build=False, **kw):
+    """Obit's CPU DFT (restated) applied in place to a copy of uv.buffer;
+    cpu_build runs the CPU-baseline build of the same code."""
     buf = np.array(uv.buffer if buffer is None else buffer, dtype=np.float32, copy=True, order="C")
     d, keep = dft_desc(uv, sky, nrow=buf.shape[0], **kw)
-    load().orc_dft_obit_threaded(C.byref(d), _p(buf), nthreads)
+    (load_cpu() if cpu_build else load()).orc_dft_obit_threaded(C.byref(d), _p(buf), nthreads)
     return buf
 
 
@@ -270,12 +291,13 @@ def grid_obit(uv, facet, flip=True, **kw):
     return grid
 
 
-def grid_obit_threaded(uv, facet, nGpI=4, nthreads=4, **kw):
-    """ObitThreadGrid decomposition; returns complex64 [ny, nx/2+1] merged."""
+def grid_obit_threaded(uv, facet, nGpI=4, nthreads=4, cpu_build=False, **kw):
+    """ObitThreadGrid decomposition; returns complex64 [ny, nx/2+1] merged.
+    cpu_build runs the CPU-baseline build of the same code."""
     g, keep = grid_desc(uv, facet, **kw)
     buf = np.ascontiguousarray(uv.buffer, np.float32)
     out = np.zeros((facet.ny, facet.nx // 2 + 1, 2), np.float32)
-    load().orc_grid_obit_threaded(C.byref(g), _p(buf), nGpI, nthreads, _p(out))
+    (load_cpu() if cpu_build else load()).orc_grid_obit_threaded(C.byref(g), _p(buf), nGpI, nthreads, _p(out))
     return out[..., 0] + 1j * out[..., 1]
 
 
