@@ -4,10 +4,10 @@
 // the same C entry points (ObitCUDAGrid.h) plus resident extensions
 // (ObitHIPVis.h).  Per-sample arithmetic restates Obit's CPU gridder
 // (src/ObitThreadGrid.c:1263-1387 fast_prep_grid, :2045-2177 fast_rot,
-// :2002-2027 fast_grid) operation by operation with FP contraction off, so
-// cell indices, GCF taps and weighted sample values are bit-identical to
-// the CPU's (the FP32 sincos of the facet phase shift is the same Cephes
-// sequence the CPU's SSE build evaluates).
+// :2002-2027 fast_grid): cell indices and GCF taps are bit-identical to the
+// CPU's; the facet phase shift is formed in FP64 turns with an FP32
+// quadrant-reduced series, so weighted sample values agree with the CPU's
+// within a few FP32 ulps (prep_vals).
 //
 // Accumulation: every facet-plane grid is cut into TU x TV-cell tiles of
 // base cells (the first column and row of a sample's 7x7 footprint).  A
@@ -27,10 +27,12 @@
 #include "host.h"
 
 #include <hipfft/hipfft.h>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 #include "ObitCUDAUtil.h"
 #include "ObitHIPVis.h"
+#include "phasor.h"
 
 #include <algorithm>
 #include <climits>
@@ -55,6 +57,7 @@ struct FacetDev {
   int nx, ny, lrow, hw;
   float uscale, vscale;
   float shift[3];
+  double sturn[3];       // shift / 2 pi: the phase shift in turns per wavelength
   float rot[9];
   float guardu, guardv;  // wavelengths, CPU arithmetic
   float maxBL2, minBL2, bmTaper;
@@ -76,52 +79,6 @@ struct Common {
 };
 
 // ---------------- device: CPU-exact sample arithmetic ----------------
-
-// One lane of the SSE Cephes sincos the CPU evaluates (same constants and
-// operation order as the oracle's restatement; FP contraction off).
-__device__ __forceinline__ void cephes_sincos(float xin, float *s, float *c) {
-#pragma clang fp contract(off)
-  const float FOPI = 1.27323954473516f;
-  const float MDP1 = -0.78515625f, MDP2 = -2.4187564849853515625e-4f,
-              MDP3 = -3.77489497744594108e-8f;
-  const float S0 = -1.9515295891E-4f, S1 = 8.3321608736E-3f, S2 = -1.6666654611E-1f;
-  const float C0 = 2.443315711809948E-005f, C1 = -1.388731625493765E-003f,
-              C2 = 4.166664568298827E-002f;
-  unsigned xb = __float_as_uint(xin);
-  unsigned sign_sin = xb & 0x80000000u;
-  float x = __uint_as_float(xb & 0x7fffffffu);
-  float y = x * FOPI;
-  int j = (int)y;
-  j = (j + 1) & ~1;
-  y = (float)j;
-  unsigned swap_sin = ((unsigned)(j & 4)) << 29;
-  bool poly_mask = (j & 2) == 0;
-  float t1 = y * MDP1, t2 = y * MDP2, t3 = y * MDP3;
-  x = x + t1;
-  x = x + t2;
-  x = x + t3;
-  unsigned sign_cos = ((unsigned)(~(j - 2) & 4)) << 29;
-  sign_sin ^= swap_sin;
-  float z = x * x;
-  float yc = C0;
-  yc = yc * z; yc = yc + C1;
-  yc = yc * z; yc = yc + C2;
-  yc = yc * z; yc = yc * z;
-  float tmp = z * 0.5f;
-  yc = yc - tmp;
-  yc = yc + 1.0f;
-  float ys = S0;
-  ys = ys * z; ys = ys + S1;
-  ys = ys * z; ys = ys + S2;
-  ys = ys * z; ys = ys * x;
-  ys = ys + x;
-  float ysin2 = poly_mask ? ys : 0.0f;
-  float ysin1 = poly_mask ? 0.0f : yc;
-  float ys_r = ys - ysin2;
-  float yc_r = yc - ysin1;
-  *s = __uint_as_float(__float_as_uint(ysin1 + ysin2) ^ sign_sin);
-  *c = __uint_as_float(__float_as_uint(yc_r + ys_r) ^ sign_cos);
-}
 
 __device__ __forceinline__ int lround_obit(float in) {
 #pragma clang fp contract(off)
@@ -162,37 +119,12 @@ __device__ __forceinline__ void cells_at(const FacetDev &f, float uu, float vv, 
   ivrow = lround_obit(v) - f.hw + f.ny / 2;
 }
 
-// fast_rot + fast_prep_grid for one (row, channel) of one facet, given
-// the channel's frequency factor and (re, im, weight).
-__device__ __forceinline__ Sample prep_vals(const FacetDev &f, const Common &cm, float uu, float vv,
-                                            float ww, int ich, float fsc, float vre, float vim,
-                                            float vwt) {
+// fast_prep_grid of a rotated sample: validity, taper, weighting, cells
+// and GCF taps, in the CPU's FP32 operations (no contraction).
+__device__ __forceinline__ Sample prep_cells(const FacetDev &f, const Common &cm, float u, float v,
+                                             int ich, float vr, float vi, float vwt) {
 #pragma clang fp contract(off)
   Sample s;
-  const float phaseSign = (uu <= 0.0f) ? -1.0f : 1.0f;
-  const float freqFact = phaseSign * fsc;
-  float u = uu * freqFact, v = vv * freqFact, w = ww * freqFact;
-  // fast_rot: phase shift (double phase, reduced, FP32 Cephes sincos)
-  float vr, vi;
-  if (f.doBeam) {
-    vr = 1.0f; vi = 0.0f;
-  } else {
-    const double twopi = 2 * 3.14159265358979323846;
-    const double itwopi = 1.0 / (2 * 3.14159265358979323846);
-    const double p0 = (double)u * (double)f.shift[0];
-    const double p1 = (double)v * (double)f.shift[1];
-    const double p2 = (double)w * (double)f.shift[2];
-    const double dphase = (p0 + p1) + p2;
-    const int iphase = (int)(dphase * itwopi);
-    const float ph = (float)(dphase - (double)iphase * twopi);
-    float sn, cs;
-    cephes_sincos(ph, &sn, &cs);
-    const float vvr = vre;
-    const float vvi = vim * phaseSign;
-    vr = vvr * cs - vvi * sn;
-    vi = vvr * sn + vvi * cs;
-  }
-  // fast_prep_grid
   float vw = vwt;
   bool want = (vw > 0.0f) && (u < f.guardu) && (fabsf(v) < f.guardv);
   const float BL2 = u * u + v * v;
@@ -217,18 +149,58 @@ __device__ __forceinline__ Sample prep_vals(const FacetDev &f, const Common &cm,
   v = v * f.vscale;
   const int iu = (int)((double)u + 0.5);
   const int iv = lround_obit(v);
-  const int N = cm.convNperCell;
+  const float N = (float)cm.convNperCell, hN = 0.5f * N;
   const float du = (float)iu - u;
   const float dv = (float)iv - v;
-  s.itu = N + lround_obit((float)((double)N * ((double)du - 0.5)));
-  s.itv = N + lround_obit((float)((double)N * ((double)dv - 0.5)));
-  if (s.itu < 0 || s.itu > N || s.itv < 0 || s.itv > N) {  // only for non-positive frequencies
+  s.itu = cm.convNperCell + lround_obit(fmaf(N, du, -hN));
+  s.itv = cm.convNperCell + lround_obit(fmaf(N, dv, -hN));
+  if (s.itu < 0 || s.itu > cm.convNperCell || s.itv < 0 || s.itv > cm.convNperCell) {
+    // only for non-positive frequencies
     s.iu = f.hw; s.iv = f.ny / 2; s.itu = 0; s.itv = 0; s.vr = 0.0f; s.vi = 0.0f;
     return s;
   }
   s.iu = iu;
   s.iv = iv - f.hw + f.ny / 2;
   return s;
+}
+
+// fast_rot + fast_prep_grid for one (row, channel) of one facet, given
+// the channel's frequency factor and (re, im, weight).
+//
+// Cell indices and GCF taps are bit-identical to the CPU's: its FP32
+// products are repeated unfused, its double-promoted roundings kept where
+// they matter (iu, iv and the taps' lround of a float), and its tap
+// fraction (float)(N * ((double)du - 0.5)) formed as fmaf(N, du, -N/2),
+// which rounds the same exact value once (checked for every float du in
+// (-0.5, 0.5]).  The facet phase shift is formed in turns in FP64 from the
+// same FP32 u, v, w (the CPU forms it in radians in FP64 and rounds the
+// reduced angle to FP32 for a Cephes sincos); here the reduced turn is
+// rounded to FP32 and goes through a quadrant reduction and Taylor
+// series, so the weighted value differs from the CPU's by a few FP32
+// ulps of the phase, within both evaluations' own error.
+__device__ __forceinline__ Sample prep_vals(const FacetDev &f, const Common &cm, float uu, float vv,
+                                            float ww, int ich, float fsc, float vre, float vim,
+                                            float vwt) {
+  float u, v, w;
+  {
+#pragma clang fp contract(off)
+    const float phaseSign = (uu <= 0.0f) ? -1.0f : 1.0f;
+    const float freqFact = phaseSign * fsc;
+    u = uu * freqFact; v = vv * freqFact; w = ww * freqFact;
+  }
+  // fast_rot: phase shift of the facet
+  float vr, vi;
+  if (f.doBeam) {
+    vr = 1.0f; vi = 0.0f;
+  } else {
+    const double t = (double)u * f.sturn[0] + (double)v * f.sturn[1] + (double)w * f.sturn[2];
+    float sn, cs;
+    sincos_turns_poly((float)(t - rint(t)), &sn, &cs);
+    const float vvi = (uu <= 0.0f) ? -vim : vim;
+    vr = vre * cs - vvi * sn;
+    vi = vre * sn + vvi * cs;
+  }
+  return prep_cells(f, cm, u, v, ich, vr, vi, vwt);
 }
 
 __device__ Sample prep_sample(const FacetDev &f, const Common &cm, const float *row, float uu,
@@ -266,48 +238,99 @@ __global__ void k_uvw_hash(const float *vis, long long nrow, int lenvis, unsigne
   if ((threadIdx.x & 63) == 0 && h) atomicAdd(out, h);
 }
 
-// mode 0: count tracks per tile of base cells; mode 1: write row indices
-// into lists.  One thread per (row, facet): the row is rotated once and
-// bounded for every plane.
-__global__ void k_bin(const float *vis, long long nrow, const FacetDev *facets, int nf,
-                      Common cm, int *counts, int *cursor, const long long *offsets, int *lists,
-                      int mode) {
+// Every (plane, tile of base cells) a track can reach, with the window
+// block of the tile it enters first: fn(L, wkey).  A track's base cells
+// are bounded by the cells at the plane's lowest and highest frequency
+// (cells are monotonic in frequency); it enters a tile near the cell of
+// the plane's first channel, which clamped into the tile gives the 8x2
+// block key (wkey = block row * 8 + block column, < 80).
+template <typename Fn>
+__device__ __forceinline__ void track_tiles(const FacetDev &f, const Common &cm, float uu, float vv,
+                                            Fn fn) {
+  const int ncol = f.lrow / 2;
+  for (int p = 0; p < cm.nplane; ++p) {
+    if (cm.pcount[p] <= 0) continue;
+    int iu0, iv0, iu1, iv1, iuf, ivf;
+    cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
+    cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
+    cells_at(f, uu, vv, cm.freq[cm.pchan[(size_t)p * cm.maxpc]], iuf, ivf);
+    int cA = min(iu0, iu1), cB = max(iu0, iu1);
+    int rA = min(iv0, iv1), rB = max(iv0, iv1);
+    cA = max(cA, 0); rA = max(rA, 0);
+    cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
+    if (cA > cB || rA > rB) continue;
+    const int tu0 = cA / TU, tu1 = cB / TU, tv0 = rA / TV, tv1 = rB / TV;
+    const long long base = f.listBase + (long long)p * f.ntu * f.ntv;
+    for (int tv = tv0; tv <= tv1; ++tv)
+      for (int tu = tu0; tu <= tu1; ++tu) {
+        const int cu = min(max(iuf - tu * TU, 0), TU - 1);
+        const int cv = min(max(ivf - tv * TV, 0), TV - 1);
+        fn(base + (long long)tv * f.ntu + tu, (unsigned)((cv >> 1) * (TU / 8) + (cu >> 3)));
+      }
+  }
+}
+
+// Binning, pass 1: the number of (plane, tile) entries of every track of
+// every facet, one thread per (row, facet), facet fastest.
+__global__ void k_bin_count(const float *vis, long long nrow, const FacetDev *facets, int nf, Common cm,
+                            unsigned *cnt) {
   const long long total = nrow * (long long)nf;
   for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
        it += (long long)gridDim.x * blockDim.x) {
-    // facet fastest: a wave's lanes read one or two rows and their
-    // counter atomics fall on different facets' lists
     const int fi = (int)(it % nf);
     const long long r = it / nf;
     const FacetDev &f = facets[fi];
-    const float *row = vis + r * (long long)cm.lenvis;
     float uu, vv, ww;
-    rotate_uvw(f, row, uu, vv, ww);
-    const int ncol = f.lrow / 2;
-    for (int p = 0; p < cm.nplane; ++p) {
-      if (cm.pcount[p] <= 0) continue;
-      int iu0, iv0, iu1, iv1;
-      cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
-      cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
-      int cA = min(iu0, iu1), cB = max(iu0, iu1);
-      int rA = min(iv0, iv1), rB = max(iv0, iv1);
-      cA = max(cA, 0); rA = max(rA, 0);
-      cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
-      if (cA > cB || rA > rB) continue;
-      const int tu0 = cA / TU, tu1 = cB / TU, tv0 = rA / TV, tv1 = rB / TV;
-      const long long base = f.listBase + (long long)p * f.ntu * f.ntv;
-      for (int tv = tv0; tv <= tv1; ++tv)
-        for (int tu = tu0; tu <= tu1; ++tu) {
-          const long long L = base + (long long)tv * f.ntu + tu;
-          if (mode == 0) {
-            atomicAdd(&counts[L], 1);
-          } else {
-            const int pos = atomicAdd(&cursor[L], 1);
-            lists[offsets[L] + pos] = (int)r;
-          }
-        }
-    }
+    rotate_uvw(f, vis + r * (long long)cm.lenvis, uu, vv, ww);
+    unsigned n = 0;
+    track_tiles(f, cm, uu, vv, [&](long long, unsigned) { ++n; });
+    cnt[it] = n;
   }
+}
+
+// Binning, pass 2: every entry as a sort key (list << 7 | window block)
+// and its row, written at the thread's scanned offset.  Sorting the pairs
+// by key gives each tile's list ordered by the window its tracks enter
+// first, then by row (the sort is stable and the emission ordered).
+__global__ void k_bin_emit(const float *vis, long long nrow, const FacetDev *facets, int nf, Common cm,
+                           const unsigned long long *off, unsigned *keys, int *rows) {
+  const long long total = nrow * (long long)nf;
+  for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
+       it += (long long)gridDim.x * blockDim.x) {
+    const int fi = (int)(it % nf);
+    const long long r = it / nf;
+    const FacetDev &f = facets[fi];
+    float uu, vv, ww;
+    rotate_uvw(f, vis + r * (long long)cm.lenvis, uu, vv, ww);
+    unsigned long long o = off[it];
+    track_tiles(f, cm, uu, vv, [&](long long L, unsigned wkey) {
+      keys[o] = ((unsigned)L << 7) | wkey;
+      rows[o] = (int)r;
+      ++o;
+    });
+  }
+}
+
+// List extents from the sorted keys: first[L] at a list's first entry and
+// last[L] one past its last (both zero for empty lists).
+__global__ void k_list_bounds(const unsigned *keys, long long n, long long *first, long long *last) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const unsigned L = keys[i] >> 7;
+    if (i == 0 || (keys[i - 1] >> 7) != L) first[L] = i;
+    if (i == n - 1 || (keys[i + 1] >> 7) != L) last[L] = i + 1;
+  }
+}
+
+__global__ void k_list_counts(const long long *first, const long long *last, long long nl, int *counts) {
+  for (long long L = (long long)blockIdx.x * blockDim.x + threadIdx.x; L < nl;
+       L += (long long)gridDim.x * blockDim.x)
+    counts[L] = (int)(last[L] - first[L]);
+}
+
+__global__ void k_zero64(long long *a, long long *b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) { a[i] = 0; b[i] = 0; }
 }
 
 // ---------------- tile gridding ----------------
@@ -677,9 +700,15 @@ struct GpuState {
   std::vector<FacetDev> h_facets;
   int *d_listFacet = nullptr;
   long long nlist = 0;
-  int *d_counts = nullptr, *d_cursor = nullptr, *d_lists = nullptr;
+  int *d_counts = nullptr, *d_lists = nullptr;
   long long *d_offsets = nullptr;  // first entry of every list (64-bit: passes may exceed 2^31 entries)
+  long long *d_lend = nullptr;     // one past the last entry of every list
   long long listCap = 0;
+  unsigned *d_tcnt = nullptr;      // entries per (row, facet) track
+  unsigned long long *d_toff = nullptr;  // their exclusive scan
+  long long tcap = 0;
+  unsigned *d_keys = nullptr, *d_keys2 = nullptr;  // sort keys, in and out
+  int *d_rows = nullptr;           // rows before the sort (d_lists after it)
   int2 *d_items = nullptr;
   long long itemCap = 0;
   long long *d_itemOff = nullptr;  // first work item of every list (exclusive scan of chunk counts)
@@ -792,7 +821,10 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
       return fail(kInvalid, "facet %d: %d x %d grid outside 2..65536 x 2..32768", gs->facets[k], fi->nx,
                   fi->ny);
     f.uscale = fi->uscale; f.vscale = fi->vscale;
-    for (int j = 0; j < 3; ++j) f.shift[j] = fi->shift[j];
+    for (int j = 0; j < 3; ++j) {
+      f.shift[j] = fi->shift[j];
+      f.sturn[j] = (double)fi->shift[j] / (2.0 * 3.14159265358979323846);
+    }
     for (int j = 0; j < 9; ++j) f.rot[j] = fi->rotUV[j];
     // guard band in wavelengths as src/ObitThreadGrid.c:1290-1291 forms it
     // from guardband*nx (src/ObitThreadGrid.c:328-329), in float
@@ -879,10 +911,14 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   cm.pfmin = gs->d_pf; cm.pfmax = gs->d_pf + nplane;
   cm.maxpc = maxpc;
   cm.gcf = gs->d_gcf;
-  // list buffers sized for the lists
-  (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets);
+  // list buffers sized for the lists; sort keys hold the list index in
+  // 25 bits above the 7-bit window block
+  if (gs->nlist >= (1LL << 25))
+    return fail(kInvalid, "%lld tile lists (facets x planes x tiles) exceed 2^25", gs->nlist);
+  (void)hipFree(gs->d_counts); (void)hipFree(gs->d_lend); (void)hipFree(gs->d_offsets);
+  gs->d_counts = nullptr; gs->d_lend = nullptr; gs->d_offsets = nullptr;
   OBIT_HIP_TRY(hipMalloc(&gs->d_counts, sizeof(int) * (gs->nlist + 1)));
-  OBIT_HIP_TRY(hipMalloc(&gs->d_cursor, sizeof(int) * (gs->nlist + 1)));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_lend, sizeof(long long) * (gs->nlist + 1)));
   OBIT_HIP_TRY(hipMalloc(&gs->d_offsets, sizeof(long long) * (gs->nlist + 1)));
   (void)hipFree(gs->d_itemOff);
   OBIT_HIP_TRY(hipMalloc(&gs->d_itemOff, sizeof(long long) * (gs->nlist + 1)));
@@ -895,6 +931,9 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
 // List sizes as 64-bit entry counts and as kChunk-track work-item counts.
 struct ToEntries {
   __host__ __device__ long long operator()(int n) const { return (long long)n; }
+};
+struct ToEntriesU {
+  __host__ __device__ unsigned long long operator()(unsigned n) const { return (unsigned long long)n; }
 };
 struct ToChunks {
   __host__ __device__ long long operator()(int n) const { return (long long)((n + kChunk - 1) / kChunk); }
@@ -913,6 +952,17 @@ __global__ __launch_bounds__(256) void k_items(const int *counts, const long lon
 }
 
 int launch_taps(GpuState *gs, const float *vis);
+
+// Scan / sort scratch of at least need bytes.
+hipError_t ensure_tmp(GpuState *gs, size_t need) {
+  if (need <= gs->scanTmpBytes) return hipSuccess;
+  (void)hipFree(gs->d_scanTmp);
+  gs->d_scanTmp = nullptr;
+  gs->scanTmpBytes = 0;
+  hipError_t e = hipMalloc(&gs->d_scanTmp, need);
+  if (e == hipSuccess) gs->scanTmpBytes = need;
+  return e;
+}
 
 // Checksum of the uvw of nrow rows (k_uvw_hash), read back.
 int uvw_hash(GpuState *gs, const float *vis, long long nrow, unsigned long long &h) {
@@ -950,60 +1000,93 @@ int grid_device(GpuState *gs, const float *vis, long long nrow, bool reuse) {
   }
   gs->listsValid = false;
   if ((long long)nrow * gs->cm.nplane * nf > (1LL << 40)) return fail(kInvalid, "too many tracks");
-  {
-    const int nbz = (int)std::min<long long>((nl + 1 + 255) / 256, 65535);
-    hipLaunchKernelGGL(k_zero2, dim3(nbz), dim3(256), 0, gs->stream, gs->d_counts, gs->d_cursor, nl + 1);
-    OBIT_HIP_TRY(hipGetLastError());
-  }
   if (!gs->d_walk) OBIT_HIP_TRY(hipMalloc(&gs->d_walk, 2 * sizeof(unsigned long long)));
   hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_walk,
                      (int *)gs->d_walk + 2, 2LL);
   OBIT_HIP_TRY(hipGetLastError());
   const long long ntr = nrow * (long long)nf;
+  if (ntr + 1 > gs->tcap) {
+    (void)hipFree(gs->d_tcnt); (void)hipFree(gs->d_toff);
+    gs->d_tcnt = nullptr; gs->d_toff = nullptr;
+    OBIT_HIP_TRY(hipMalloc(&gs->d_tcnt, sizeof(unsigned) * (size_t)(ntr + 1)));
+    OBIT_HIP_TRY(hipMalloc(&gs->d_toff, sizeof(unsigned long long) * (size_t)(ntr + 1)));
+    gs->tcap = ntr + 1;
+  }
   const int nbB = (int)std::min<long long>((ntr + 255) / 256, (long long)gs->nCU * 32);
-  hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
-                     gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 0);
+  // pass 1: entries per track; the slot after the last track is zero so
+  // the scan's last element is the total
+  hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, gs->stream, (int *)gs->d_tcnt + ntr,
+                     (int *)gs->d_tcnt + ntr, 1LL);
+  hipLaunchKernelGGL(k_bin_count, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf, gs->cm,
+                     gs->d_tcnt);
   OBIT_HIP_TRY(hipGetLastError());
-  // list offsets and work items on the device: exclusive scans of the list
-  // sizes and of their kChunk-track chunk counts (the entry after the last
-  // list is zero, so each scan's last element is its total); only the two
-  // totals come back, to size the lists and the launch
   {
-    const auto entries = rocprim::make_transform_iterator(gs->d_counts, ToEntries());
-    const auto chunks = rocprim::make_transform_iterator(gs->d_counts, ToChunks());
-    size_t need = 0, need2 = 0;
-    OBIT_HIP_TRY(rocprim::exclusive_scan(nullptr, need, entries, gs->d_offsets, 0LL, (size_t)nl + 1,
-                                         rocprim::plus<long long>(), gs->stream));
-    OBIT_HIP_TRY(rocprim::exclusive_scan(nullptr, need2, chunks, gs->d_itemOff, 0LL, (size_t)nl + 1,
-                                         rocprim::plus<long long>(), gs->stream));
-    need = std::max(need, need2);
-    if (need > gs->scanTmpBytes) {
-      (void)hipFree(gs->d_scanTmp);
-      gs->d_scanTmp = nullptr;
-      OBIT_HIP_TRY(hipMalloc(&gs->d_scanTmp, need));
-      gs->scanTmpBytes = need;
-    }
-    OBIT_HIP_TRY(rocprim::exclusive_scan(gs->d_scanTmp, need, entries, gs->d_offsets, 0LL, (size_t)nl + 1,
-                                         rocprim::plus<long long>(), gs->stream));
-    OBIT_HIP_TRY(rocprim::exclusive_scan(gs->d_scanTmp, need, chunks, gs->d_itemOff, 0LL, (size_t)nl + 1,
-                                         rocprim::plus<long long>(), gs->stream));
-    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_tot, gs->d_offsets + nl, sizeof(long long), hipMemcpyDeviceToHost,
-                                gs->stream));
-    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_tot + 1, gs->d_itemOff + nl, sizeof(long long), hipMemcpyDeviceToHost,
+    size_t need = 0;
+    const auto cnt64 = rocprim::make_transform_iterator(gs->d_tcnt, ToEntriesU());
+    OBIT_HIP_TRY(rocprim::exclusive_scan(nullptr, need, cnt64, gs->d_toff, 0ULL, (size_t)ntr + 1,
+                                         rocprim::plus<unsigned long long>(), gs->stream));
+    OBIT_HIP_TRY(ensure_tmp(gs, need));
+    OBIT_HIP_TRY(rocprim::exclusive_scan(gs->d_scanTmp, need, cnt64, gs->d_toff, 0ULL, (size_t)ntr + 1,
+                                         rocprim::plus<unsigned long long>(), gs->stream));
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_tot, gs->d_toff + ntr, sizeof(long long), hipMemcpyDeviceToHost,
                                 gs->stream));
     OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
   }
   const long long total = gs->h_tot[0];
-  const long long nitems = gs->h_tot[1];
-  gs->nitems = nitems;
-  if (total < 0 || nitems < 0 || nitems > total) return fail(kHipError, "list scan gave %lld entries, %lld items", total, nitems);
-  if ((long long)total > gs->listCap) {
-    (void)hipFree(gs->d_lists);
-    gs->d_lists = nullptr;
-    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int) * (size_t)std::max<long long>(total, 1));
-    if (e != hipSuccess) return fail(kNoMemory, "track lists (%lld): %s", total, hipGetErrorString(e));
+  if (total < 0) return fail(kHipError, "binning scan gave %lld entries", total);
+  if (total > gs->listCap) {
+    (void)hipFree(gs->d_lists); (void)hipFree(gs->d_keys); (void)hipFree(gs->d_keys2); (void)hipFree(gs->d_rows);
+    gs->d_lists = nullptr; gs->d_keys = nullptr; gs->d_keys2 = nullptr; gs->d_rows = nullptr;
+    gs->listCap = 0;
+    const size_t m = (size_t)std::max<long long>(total, 1);
+    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int) * m);
+    if (e == hipSuccess) e = hipMalloc(&gs->d_rows, sizeof(int) * m);
+    if (e == hipSuccess) e = hipMalloc(&gs->d_keys, sizeof(unsigned) * m);
+    if (e == hipSuccess) e = hipMalloc(&gs->d_keys2, sizeof(unsigned) * m);
+    if (e != hipSuccess) return fail(kNoMemory, "track lists (%lld entries): %s", total, hipGetErrorString(e));
     gs->listCap = total;
   }
+  // pass 2: keys and rows, then the stable sort by (list, window block)
+  hipLaunchKernelGGL(k_bin_emit, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf, gs->cm,
+                     gs->d_toff, gs->d_keys, gs->d_rows);
+  OBIT_HIP_TRY(hipGetLastError());
+  int endBit = 7;
+  while ((1LL << (endBit - 7)) < nl) ++endBit;
+  if (total > 0) {
+    size_t need = 0;
+    OBIT_HIP_TRY(rocprim::radix_sort_pairs(nullptr, need, gs->d_keys, gs->d_keys2, gs->d_rows, gs->d_lists,
+                                           (size_t)total, 0, endBit, gs->stream));
+    OBIT_HIP_TRY(ensure_tmp(gs, need));
+    OBIT_HIP_TRY(rocprim::radix_sort_pairs(gs->d_scanTmp, need, gs->d_keys, gs->d_keys2, gs->d_rows,
+                                           gs->d_lists, (size_t)total, 0, endBit, gs->stream));
+  }
+  // list extents and sizes, then the work items: an exclusive scan of the
+  // lists' kChunk-track chunk counts (the entry after the last list is
+  // zero, so the scan's last element is the number of items)
+  {
+    const int nbz = (int)std::min<long long>((nl + 1 + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_zero64, dim3(nbz), dim3(256), 0, gs->stream, gs->d_offsets, gs->d_lend, nl + 1);
+    const int nbk = (int)std::min<long long>((total + 255) / 256, (long long)gs->nCU * 32);
+    if (total > 0)
+      hipLaunchKernelGGL(k_list_bounds, dim3(nbk), dim3(256), 0, gs->stream, gs->d_keys2, total, gs->d_offsets,
+                         gs->d_lend);
+    hipLaunchKernelGGL(k_list_counts, dim3(nbz), dim3(256), 0, gs->stream, gs->d_offsets, gs->d_lend, nl + 1,
+                       gs->d_counts);
+    OBIT_HIP_TRY(hipGetLastError());
+    const auto chunks = rocprim::make_transform_iterator(gs->d_counts, ToChunks());
+    size_t need = 0;
+    OBIT_HIP_TRY(rocprim::exclusive_scan(nullptr, need, chunks, gs->d_itemOff, 0LL, (size_t)nl + 1,
+                                         rocprim::plus<long long>(), gs->stream));
+    OBIT_HIP_TRY(ensure_tmp(gs, need));
+    OBIT_HIP_TRY(rocprim::exclusive_scan(gs->d_scanTmp, need, chunks, gs->d_itemOff, 0LL, (size_t)nl + 1,
+                                         rocprim::plus<long long>(), gs->stream));
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_tot + 1, gs->d_itemOff + nl, sizeof(long long), hipMemcpyDeviceToHost,
+                                gs->stream));
+    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+  }
+  const long long nitems = gs->h_tot[1];
+  gs->nitems = nitems;
+  if (nitems < 0 || nitems > total) return fail(kHipError, "list scan gave %lld entries, %lld items", total, nitems);
   if (nitems > gs->itemCap) {
     (void)hipFree(gs->d_items);
     gs->d_items = nullptr;
@@ -1016,9 +1099,6 @@ int grid_device(GpuState *gs, const float *vis, long long nrow, bool reuse) {
                        gs->d_items);
     OBIT_HIP_TRY(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_bin, dim3(nbB), dim3(256), 0, gs->stream, vis, nrow, gs->d_facets, nf,
-                     gs->cm, gs->d_counts, gs->d_cursor, gs->d_offsets, gs->d_lists, 1);
-  OBIT_HIP_TRY(hipGetLastError());
   {
     int rc = uvw_hash(gs, vis, nrow, gs->listsHash);
     if (rc) return rc;
@@ -1215,7 +1295,9 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     }
     (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
-    (void)hipFree(gs->d_counts); (void)hipFree(gs->d_cursor); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
+    (void)hipFree(gs->d_counts); (void)hipFree(gs->d_lend); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
+    (void)hipFree(gs->d_tcnt); (void)hipFree(gs->d_toff); (void)hipFree(gs->d_keys); (void)hipFree(gs->d_keys2);
+    (void)hipFree(gs->d_rows);
     (void)hipFree(gs->d_items); (void)hipFree(gs->d_vis); (void)hipFree(gs->d_walk); (void)hipFree(gs->d_hash);
     (void)hipFree(gs->d_itemOff); (void)hipFree(gs->d_scanTmp);
     if (gs->h_tot) (void)hipHostFree(gs->h_tot);

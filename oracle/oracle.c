@@ -1129,3 +1129,44 @@ void orc_grid_obit_threaded(const orc_grid_desc *g, const float *buffer,
     free(jb[j].grid);
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Arithmetic identities the device gridder relies on                  */
+/* ------------------------------------------------------------------ */
+
+/* The device forms the CPU's GCF tap fraction (float)(N * ((double)du -
+ * 0.5)) (src/ObitThreadGrid.c:1366-1376) as fmaf(N, du, -N/2).  Both are
+ * roundings of the same real number: this counts the floats du in
+ * (-0.5, 0.5] whose bit patterns lie in [lo, hi) for which they differ. */
+typedef struct { uint64_t lo, hi; float N; long bad; } fma_job;
+static void *fma_run(void *a)
+{
+  fma_job *j = (fma_job *)a;
+  long bad = 0;
+  for (uint64_t u = j->lo; u < j->hi; u++) {
+    fbits b; b.u = (uint32_t)u;
+    float du = b.f;
+    if (!(du > -0.5f && du <= 0.5f)) continue;
+    float x = (float)((double)j->N * ((double)du - 0.5));
+    float y = fmaf(j->N, du, -0.5f * j->N);
+    fbits bx, by; bx.f = x; by.f = y;
+    if (bx.u != by.u) bad++;
+  }
+  j->bad = bad;
+  return NULL;
+}
+long orc_check_tap_fraction(float N, int nthreads)
+{
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 64) nthreads = 64;
+  pthread_t th[64]; fma_job jb[64];
+  const uint64_t total = 0x100000000ULL, per = total / nthreads;
+  for (int i = 0; i < nthreads; i++) {
+    jb[i].lo = i * per; jb[i].hi = (i == nthreads - 1) ? total : (i + 1) * per;
+    jb[i].N = N; jb[i].bad = 0;
+    pthread_create(&th[i], NULL, fma_run, &jb[i]);
+  }
+  long bad = 0;
+  for (int i = 0; i < nthreads; i++) { pthread_join(th[i], NULL); bad += jb[i].bad; }
+  return bad;
+}

@@ -133,6 +133,11 @@ void orc_grid_merge(const float *grid, int nx, int ny, int convWidth, float *out
 void orc_grid_obit_threaded(const orc_grid_desc *g, const float *buffer,
                             int nGpI, int nthreads, float *out);
 
+/* Floats du in (-0.5, 0.5] for which fmaf(N, du, -N/2) differs from the
+ * CPU's (float)(N * ((double)du - 0.5)) (0 when the device's tap-fraction
+ * form is exact). */
+long orc_check_tap_fraction(float N, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

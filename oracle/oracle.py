@@ -107,6 +107,8 @@ def _declare(lib):
         lib.orc_grid_flip_fp64.argtypes = [P, C.c_int, C.c_int, C.c_int]
         lib.orc_grid_merge.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
         lib.orc_grid_obit_threaded.argtypes = [C.POINTER(GridDesc), P, C.c_int, C.c_int, P]
+        lib.orc_check_tap_fraction.argtypes = [C.c_float, C.c_int]
+        lib.orc_check_tap_fraction.restype = C.c_long
     return lib
 
 

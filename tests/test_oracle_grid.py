@@ -313,3 +313,10 @@ def test_fft2image_recovers_an_offset_point_source(oracle):
     assert abs(peak - 1.0) < 2e-3
     xc = oracle.grid_corr_fn(nx, nx // 2 + 1, cf)
     assert peak / (xc[py] * xc[px]) < 0.99  # the uncorrected image is tapered there
+
+
+def test_tap_fraction_fma_form_is_exact(oracle):
+    """The device forms the CPU's GCF tap fraction (float)(N*((double)du -
+    0.5)) (ObitThreadGrid.c:1366-1376) as one fmaf(N, du, -N/2): for N = 100
+    the two agree on every float du in (-0.5, 0.5] (exhaustive)."""
+    assert oracle.load().orc_check_tap_fraction(100.0, 8) == 0
