@@ -59,6 +59,20 @@ int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq
  * freqScale is read again whenever its pointer changes or a table has been
  * uploaded through ObitCUDAUtilHost2GPU / ObitCUDAUtilHostAny2GPU since. */
 int ObitHIPSkyModelDFTGetPath(GPUInfo *gpuInfo, int *path);
+/* CalcResident's operation on nvis rows at d_vis (the caller's device
+ * buffer) with the component table d_model (the caller's device memory,
+ * ncomp rows of h_modelInfo->size floats), without allocating anything.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The layout,
+ * selection, model type and operation are those DFTInit / DFTSetMod set up;
+ * the per-IF frequency lines and the prepared component table go to a
+ * static device scratch of 32 MiB per GPU (838,809 components at most;
+ * more is refused with OBIT_HIP_EINVAL, as are spectral model types, more
+ * than 64 selected IFs and channel tables that are not a line within each
+ * IF).  Asynchronous on ObitHIPSkyModelDFTStream; calls that overlap on one
+ * GPU share the scratch and race. */
+int ObitHIPSkyModelDFTCalcNoAlloc(GPUInfo *gpuInfo, const float *d_model, int ncomp, float *d_vis,
+                                  long long nvis);
 /* Reference frequencies of the component spectra in FP64 (Hz): one value
  * (parametrised spectra: the image reference frequency) or nSpec values
  * (tabulated spectra: the coarse-bin frequencies).  Without this call they

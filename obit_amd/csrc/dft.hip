@@ -476,6 +476,23 @@ void destroy_graph(State *st) {
   st->graphKey = 0;
 }
 
+// Host copy of the FP32 freqScale table: read again when it is another
+// table, has been uploaded again through the copy utilities, is longer than
+// the cached copy, or always.
+int refresh_fscale(State *st, const GPUVisInfo *vi, bool always) {
+  const int nfreq = vi->nchan * vi->nIF;
+  const unsigned long long gen = upload_generation();
+  if (always || st->fscaleHost.size() < (size_t)nfreq || st->fscalePtr != vi->freqScale ||
+      st->fscaleGen != gen) {
+    if (!vi->freqScale) return fail(kInvalid, "no frequency information (freqScale is NULL)");
+    st->fscaleHost.resize(nfreq);
+    OBIT_HIP_TRY(hipMemcpy(st->fscaleHost.data(), vi->freqScale, sizeof(float) * nfreq, hipMemcpyDeviceToHost));
+    st->fscalePtr = vi->freqScale;
+    st->fscaleGen = gen;
+  }
+  return kOK;
+}
+
 // Frequency of every selected channel relative to nu0, as a line per IF:
 // f(c) = fbase + (c - chanb) * fdel.  Exact FP64 values when the caller
 // gave them, else a least-squares line through the FP32 freqScale.
@@ -492,17 +509,8 @@ int frequency_lines(State *st, const GPUVisInfo *vi, std::vector<double> &fb,
       return fail(kInvalid, "frequency table has %zu values, need %d", st->freq64.size(), nfreq);
     for (int i = 0; i < nfreq; ++i) f[i] = st->freq64[i] / st->uvRef;
   } else {
-    // re-read the table when it is another one, has been uploaded again
-    // through the copy utilities, or is longer than the cached copy
-    const unsigned long long gen = upload_generation();
-    if (st->fscaleHost.size() < (size_t)nfreq || st->fscalePtr != vi->freqScale || st->fscaleGen != gen) {
-      if (!vi->freqScale) return fail(kInvalid, "no frequency information (freqScale is NULL)");
-      st->fscaleHost.resize(nfreq);
-      OBIT_HIP_TRY(hipMemcpy(st->fscaleHost.data(), vi->freqScale, sizeof(float) * nfreq,
-                             hipMemcpyDeviceToHost));
-      st->fscalePtr = vi->freqScale;
-      st->fscaleGen = gen;
-    }
+    const int rc = refresh_fscale(st, vi, false);
+    if (rc) return rc;
     for (int i = 0; i < nfreq; ++i) f[i] = st->fscaleHost[i];
   }
   st->uniform = true;
@@ -608,11 +616,11 @@ int ensure_spec_table(State *st, const GPUInfo *gi) {
 }
 
 // Fill a kernel descriptor for nrow rows at vis; uploads the per-IF lines.
-int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt, Desc &d) {
+// The layout, selection and operation fields of a kernel descriptor for
+// nrow rows at vis (no allocation, no frequencies, no model pointers).
+int desc_layout(const GPUInfo *gi, float *vis, long long nrow, int cpt, Desc &d) {
   const GPUVisInfo *vi = gi->h_visInfo;
   const GPUModelInfo *mi = gi->h_modelInfo;
-  if (!vi || !mi) return fail(kState, "h_visInfo / h_modelInfo not set");
-  if (!st->modelReady) return fail(kState, "no model: call ObitCUDASkyModelDFTSetMod first");
   std::memset(&d, 0, sizeof(d));
   d.vis = vis;
   d.nrow = nrow;
@@ -641,9 +649,24 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
   d.nchunk = (d.nchsel + cpt - 1) / cpt;
   d.nrowblk = (int)((nrow + kRows - 1) / kRows);
   d.nitems = (long long)d.nrowblk * d.nIFsel * d.nchunk;
+  d.op = (int)mi->opType;
+  if (d.op < 0 || d.op > 2) return fail(kInvalid, "unknown operation %d", d.op);
+  d.doSwap = mi->doSwap;
+  d.stokFactor = mi->stokFact[1];
+  return kOK;
+}
+
+// Fill a kernel descriptor for nrow rows at vis; uploads the per-IF lines.
+int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt, Desc &d) {
+  const GPUVisInfo *vi = gi->h_visInfo;
+  const GPUModelInfo *mi = gi->h_modelInfo;
+  if (!vi || !mi) return fail(kState, "h_visInfo / h_modelInfo not set");
+  if (!st->modelReady) return fail(kState, "no model: call ObitCUDASkyModelDFTSetMod first");
+  int rc = desc_layout(gi, vis, nrow, cpt, d);
+  if (rc) return rc;
   // per-IF frequency lines, then every selected channel's own value
   std::vector<double> fb, fdl, fch;
-  int rc = frequency_lines(st, vi, fb, fdl, &fch);
+  rc = frequency_lines(st, vi, fb, fdl, &fch);
   if (rc) return rc;
   const int nfq = 2 * d.nIFsel + d.nIFsel * d.nchsel;
   if (nfq > st->freqCap) {
@@ -673,10 +696,6 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
     d.tab_nchpad = (d.nchsel + 63) / 64 * 64;
     d.tab_ldk = d.nIFsel * d.tab_nchpad;
   }
-  d.op = (int)mi->opType;
-  if (d.op < 0 || d.op > 2) return fail(kInvalid, "unknown operation %d", d.op);
-  d.doSwap = mi->doSwap;
-  d.stokFactor = mi->stokFact[1];
   return kOK;
 }
 
@@ -723,6 +742,97 @@ int run_device(State *st, const GPUInfo *gi, float *vis, long long nrow, bool fp
   return launch_split(st, d, st->cpt, st->tile, st->wgPerCU);
 }
 
+// ---- the call that allocates nothing (ObitHIPSkyModelDFTCalcNoAlloc) ----
+
+// Device scratch of the no-allocation call: the per-IF frequency lines and
+// the prepared component table live here instead of in library-allocated
+// buffers.  One copy per GPU, held until the process ends; calls on one GPU
+// share it.
+constexpr size_t kNoAllocBytes = size_t(32) << 20;
+constexpr int kNoAllocMaxIF = 64;
+constexpr size_t kNoAllocLineBytes = 2 * kNoAllocMaxIF * sizeof(double);
+constexpr size_t kNoAllocCompBytes = 3 * sizeof(double) + 4 * sizeof(float);  // x, y, z; amplitude, 3 Gaussian
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scr_mu;
+void *g_scr_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch (in host code the array's name
+// is only a stand-in): looked up once per device, with that device current.
+int scratch_ptr(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(kInvalid, "device %d outside 0..%d", device, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if (!g_scr_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(kHipError, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scr_ptr[device] = p;
+  }
+  *out = g_scr_ptr[device];
+  return kOK;
+}
+
+struct Lines {
+  double fb[kNoAllocMaxIF], fdl[kNoAllocMaxIF];
+};
+
+__global__ void k_store_lines(Lines v, int n, double *fb, double *fdl) {
+  const int i = threadIdx.x;
+  if (i < n) { fb[i] = v.fb[i]; fdl[i] = v.fdl[i]; }
+}
+
+// frequency_lines for the no-allocation call: the same arithmetic from the
+// host copies taken at SetFreq / SetMod, into fixed storage; refuses tables
+// that are not a line within each IF (the FP64 kernel needs a per-channel
+// table).
+int lines_fixed(const State *st, const GPUVisInfo *vi, Lines &L) {
+  const int nchsel = vi->chane - vi->chanb + 1, nIFsel = vi->IFe - vi->IFb + 1;
+  if (nIFsel > kNoAllocMaxIF) return fail(kInvalid, "%d IFs selected; the no-allocation call takes at most %d",
+                                          nIFsel, kNoAllocMaxIF);
+  const int nfreq = vi->nchan * vi->nIF;
+  const bool f64 = !st->freq64.empty() && st->uvRef > 0.0;
+  if (f64 && (int)st->freq64.size() < nfreq)
+    return fail(kInvalid, "frequency table has %zu values, need %d", st->freq64.size(), nfreq);
+  if (!f64 && (st->fscaleHost.size() < (size_t)nfreq || st->fscalePtr != vi->freqScale))
+    return fail(kState, "no host copy of this freqScale table: call ObitCUDASkyModelDFTSetMod first");
+  auto val = [&](int IF, int c) {
+    const int i = IF * vi->kincif + (vi->chanb + c) * vi->kincf;
+    return f64 ? st->freq64[i] / st->uvRef : (double)st->fscaleHost[i];
+  };
+  for (int is = 0; is < nIFsel; ++is) {
+    const int IF = vi->IFb + is;
+    double a = val(IF, 0), b = 0.0;
+    if (nchsel > 1) {
+      if (f64) {
+        b = (val(IF, nchsel - 1) - val(IF, 0)) / (nchsel - 1);
+      } else {  // least squares through FP32 values, as frequency_lines
+        double sx = 0, sy = 0, sxx = 0, sxy = 0;
+        for (int c = 0; c < nchsel; ++c) {
+          const double y = val(IF, c);
+          sx += c; sy += y; sxx += (double)c * c; sxy += c * y;
+        }
+        const double n = nchsel;
+        b = (n * sxy - sx * sy) / (n * sxx - sx * sx);
+        a = (sy - b * sx) / n;
+      }
+      for (int c = 0; c < nchsel; ++c) {
+        const double y = val(IF, c);
+        const double dev = std::fabs(y - (a + b * c));
+        const double tol = f64 ? 1e-12 * std::fabs(y) : 4e-7 * std::fabs(y);
+        if (dev > tol)
+          return fail(kInvalid, "channel frequencies of IF %d are not uniform: the no-allocation call runs the "
+                                "split kernel only", IF);
+      }
+    }
+    L.fb[is] = a;
+    L.fdl[is] = b;
+  }
+  return kOK;
+}
+
 }  // namespace dft
 }  // namespace obit
 
@@ -745,6 +855,14 @@ void ObitCUDASkyModelDFTInit(GPUInfo *gpuInfo) {
   hipDeviceProp_t prop;
   OBIT_HIP_CHECK(hipGetDeviceProperties(&prop, st->device));
   st->nCU = prop.multiProcessorCount;
+  {
+    // this GPU's scratch address for the no-allocation call
+    void *p = nullptr;
+    if (obit::dft::scratch_ptr(st->device, &p)) {
+      std::fprintf(stderr, "obit_hip: DFTInit: %s\n", ObitHIPLastError());
+      std::abort();
+    }
+  }
   st->fscaleHost.clear();
 }
 
@@ -781,7 +899,13 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
   OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
   st->modelReady = true;
   st->tabReady = false;
-  st->fscaleHost.clear();  // a new model may come with a new frequency table
+  // a new model may come with a new frequency table: keep the host copy of
+  // the one the descriptor names now
+  if (gpuInfo->h_visInfo && gpuInfo->h_visInfo->freqScale && st->freq64.empty() &&
+      obit::dft::refresh_fscale(st, gpuInfo->h_visInfo, true)) {
+    std::fprintf(stderr, "obit_hip: DFTSetMod: %s\n", ObitHIPLastError());
+    std::abort();
+  }
   obit::dft::destroy_graph(st);
 }
 
@@ -845,6 +969,63 @@ int ObitHIPSkyModelDFTGetPath(GPUInfo *gpuInfo, int *path) {
   if (!path) return obit::fail(obit::kInvalid, "path is NULL");
   *path = st->lastPath;
   return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTCalcNoAlloc(GPUInfo *gpuInfo, const float *d_model, int ncomp, float *d_vis,
+                                  long long nvis) {
+  using namespace obit;
+  using namespace obit::dft;
+  State *st = find_state(gpuInfo);
+  if (!st) return fail(kState, "DFTCalcNoAlloc before DFTInit");
+  if (!st->modelReady) return fail(kState, "DFTCalcNoAlloc: call ObitCUDASkyModelDFTSetMod first (model type)");
+  if (ncomp < 0 || nvis < 0) return fail(kInvalid, "negative component or row count");
+  if ((ncomp > 0 && !d_model) || (nvis > 0 && !d_vis)) return fail(kInvalid, "null device pointer");
+  if (model_is_spec(st->type))
+    return fail(kInvalid, "spectral model types need the library's spectral table: use CalcResident");
+  const size_t need = kNoAllocLineBytes + (size_t)ncomp * kNoAllocCompBytes + 4 * 256;
+  if (need > kNoAllocBytes)
+    return fail(kInvalid, "%d components need %zu bytes of scratch; the no-allocation call holds %zu (%zu components)",
+                ncomp, need, kNoAllocBytes, (kNoAllocBytes - kNoAllocLineBytes - 4 * 256) / kNoAllocCompBytes);
+  const int gOff = model_gauss_off(st->type);
+  if (gOff >= 0 && st->size < 7) return fail(kInvalid, "Gaussian model needs 7 floats per row");
+  if (nvis == 0) return kOK;
+  Desc d;
+  int rc = desc_layout(gpuInfo, d_vis, nvis, st->cpt, d);
+  if (rc) return rc;
+  Lines L;
+  rc = lines_fixed(st, gpuInfo->h_visInfo, L);
+  if (rc) return rc;
+  OBIT_HIP_TRY(hipSetDevice(st->device));
+  void *base = nullptr;
+  rc = scratch_ptr(st->device, &base);
+  if (rc) return rc;
+  // scratch layout: lines, then x, y, z (FP64), amplitudes, Gaussian terms
+  auto align = [](size_t o) { return (o + 255) & ~size_t(255); };
+  unsigned char *p = static_cast<unsigned char *>(base);
+  size_t o = 0;
+  double *fb = reinterpret_cast<double *>(p + o);
+  double *fdl = fb + kNoAllocMaxIF;
+  o = align(o + kNoAllocLineBytes);
+  double *cx = reinterpret_cast<double *>(p + o); o = align(o + sizeof(double) * ncomp);
+  double *cy = reinterpret_cast<double *>(p + o); o = align(o + sizeof(double) * ncomp);
+  double *cz = reinterpret_cast<double *>(p + o); o = align(o + sizeof(double) * ncomp);
+  float *camp = reinterpret_cast<float *>(p + o); o = align(o + sizeof(float) * ncomp);
+  float *cg = reinterpret_cast<float *>(p + o); o += 3 * sizeof(float) * ncomp;
+  if (o > kNoAllocBytes) return fail(kInvalid, "scratch layout overflow (%zu bytes)", o);
+  hipLaunchKernelGGL(k_store_lines, dim3(1), dim3(kNoAllocMaxIF), 0, st->stream, L, d.nIFsel, fb, fdl);
+  OBIT_HIP_TRY(hipGetLastError());
+  if (ncomp > 0) {
+    hipLaunchKernelGGL(k_prep_comps, dim3((ncomp + 255) / 256), dim3(256), 0, st->stream, d_model, st->size,
+                       ncomp, gOff, cx, cy, cz, camp, cg);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  d.fbase = fb;
+  d.fdel = fdl;
+  d.ncomp = ncomp;
+  d.cx = cx; d.cy = cy; d.cz = cz;
+  d.camp = camp; d.cg = cg;
+  st->lastPath = 0;
+  return launch_split(st, d, st->cpt, st->tile, st->wgPerCU);
 }
 
 int ObitHIPSkyModelDFTSetSpecRef(GPUInfo *gpuInfo, const double *refFreq, int n) {
