@@ -133,15 +133,19 @@ int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out);
 int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis,
                          int *iu, int *iv, int *itu, int *itv, float *vis);
 /* ObitUVGridFFT2Im (src/ObitUVGrid.c:673-834) on the device for every plane
- * of facet ifacet, after ObitCUDAGridFlip: trim and centre the planes, one
- * batched C2R FFT (unnormalised, as FFTW's backward transform), the
- * quadrant swap of ObitFArray2DCenter, the gridding correction of
- * GridCorrFn applied as ObitFArrayMulColRow does, and the normalisation.
- * isBeam=1: beamNorm[p] receives each plane's centre value x corrections
- * and normalises it; isBeam=0: beamNorm[p] (from the beam) normalises the
- * image.  d_image: device [nplane][ny][nx] floats.  Facets must be square
- * and even-sized (the reference's correction indexes rows and columns
- * alike).  Returns after the images are written. */
+ * of facet ifacet, after ObitCUDAGridFlip: one batched C2R FFT
+ * (unnormalised, as FFTW's backward transform) read straight from the
+ * gridder's padded half planes, then in one pass the v-half swap (as a
+ * (-1)^y factor), the quadrant swap of ObitFArray2DCenter, the gridding
+ * correction of GridCorrFn applied as ObitFArrayMulColRow does, and the
+ * normalisation.  isBeam=1: beamNorm[p] receives each plane's centre value
+ * x corrections and normalises it; isBeam=0: beamNorm[p] (from the beam)
+ * normalises the image.  d_image: device [nplane][ny][nx] floats.  Facets
+ * must be square with a side a multiple of 8 (the reference's correction
+ * indexes rows and columns alike); at most 1024 planes.  The transform
+ * consumes the facet's grid planes (as the reference's transforms its
+ * grid): read them first if they are wanted.  Returns after the images
+ * are written. */
 int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamNorm, float *d_image);
 /* Statistics of the last gridding pass on iGPU (stats has 8 entries):
  * [0] tile lists, [1] non-empty lists, [2] list entries (track x tile),

@@ -379,6 +379,7 @@ KernelFn pick_gs(bool g, bool s) {
 
 template <int P>
 KernelFn pick_shape(int cpt, int tile, bool g, bool s) {
+  if (cpt == 8) return pick_gs<8, 64, P>(g, s);  // small buffers: more work items
   if (cpt == 16) {
     if (tile == 32) return pick_gs<16, 32, P>(g, s);
     if (tile == 128) return pick_gs<16, 128, P>(g, s);
@@ -699,7 +700,7 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
   return kOK;
 }
 
-int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU) {
+int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipStream_t stream = nullptr) {
   if (d.nitems <= 0) return kOK;
   const bool g = model_gauss_off(st->type) >= 0, s = model_is_spec(st->type);
   KernelFn fn = pick_split(cpt, tile, g, s, st->prec);
@@ -707,7 +708,8 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU) {
   if (grid > d.nitems) grid = d.nitems;
   Desc dd = d;
   void *args[] = {&dd};
-  OBIT_HIP_TRY(hipLaunchKernel((const void *)fn, dim3((unsigned)grid), dim3(kRows), args, 0, st->stream));
+  OBIT_HIP_TRY(hipLaunchKernel((const void *)fn, dim3((unsigned)grid), dim3(kRows), args, 0,
+                               stream ? stream : st->stream));
   return kOK;
 }
 
@@ -910,6 +912,7 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
 }
 
 void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
+  using namespace obit::dft;
   State *st = find_state(gpuInfo);
   if (!st || !st->modelReady) {
     std::fprintf(stderr, "obit_hip: DFTCalc before DFTInit/DFTSetMod\n");
@@ -917,18 +920,79 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
   }
   OBIT_HIP_CHECK(hipSetDevice(st->device));
   const long long nvis = gpuInfo->nvis;
-  const size_t bytes = (size_t)nvis * gpuInfo->lenvis * sizeof(float);
+  const size_t rowBytes = (size_t)gpuInfo->lenvis * sizeof(float);
+  const size_t bytes = (size_t)nvis * rowBytes;
   if (nvis <= 0) return;
   if (bytes > st->stageCap) {
     if (st->stage) OBIT_HIP_CHECK(hipFree(st->stage));
     OBIT_HIP_CHECK(hipMalloc(&st->stage, bytes));
     st->stageCap = bytes;
   }
-  OBIT_HIP_CHECK(hipMemcpyAsync(st->stage, gpuInfo->h_data, bytes, hipMemcpyHostToDevice, st->stream));
-  int rc = obit::dft::run_device(st, gpuInfo, st->stage, nvis, false);
+  // channels per thread for this buffer: a persistent launch takes
+  // ceil(items / slots) rounds of items that each cost about cpt / e(cpt)
+  // (e = 1, 0.9, 0.6 for 32, 16, 8 channels: the per-chunk phasor set-up
+  // weighs more on short chunks); a 10,000-row buffer of 4 IFs x 64
+  // channels has 320 items at 32 channels (one round on a third of the
+  // GPU), 640 at 16 (one round), 1,280 at 8 (two rounds)
+  Desc d;
+  int rc = make_desc(st, gpuInfo, st->stage, nvis, 32, d);
   if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
-  OBIT_HIP_CHECK(hipMemcpyAsync(gpuInfo->h_data, st->stage, bytes, hipMemcpyDeviceToHost, st->stream));
-  OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+  const long long slots = (long long)st->nCU * st->wgPerCU;
+  int cpt = 32, tile = st->tile;
+  double best = 0.0;
+  for (int c : {32, 16, 8}) {
+    const long long items = (long long)d.nrowblk * d.nIFsel * ((d.nchsel + c - 1) / c);
+    const double cost = (double)((items + slots - 1) / slots) * c / (c == 32 ? 1.0 : c == 16 ? 0.9 : 0.6);
+    if (c == 32 || cost < best) { best = cost; cpt = c; }
+  }
+  if (cpt == 8) tile = 64;
+  d.nchunk = (d.nchsel + cpt - 1) / cpt;
+  if (!st->uniform) {
+    // the FP64 kernel, in one piece on the library's stream
+    OBIT_HIP_CHECK(hipMemcpyAsync(st->stage, gpuInfo->h_data, bytes, hipMemcpyHostToDevice, st->stream));
+    rc = run_device(st, gpuInfo, st->stage, nvis, false);
+    if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
+    OBIT_HIP_CHECK(hipMemcpyAsync(gpuInfo->h_data, st->stage, bytes, hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+    return;
+  }
+  st->lastPath = 0;
+  // the buffer in up to STREAM_COUNT row segments on the caller's streams
+  // (ObitGPUSkyModelDFTInit creates them): each segment's upload, launch
+  // and download queue on its own stream, so copies overlap the other
+  // segments' launches (as the reference's ObitCUDASkyModel.cu:804-963)
+  int nseg = 0;
+  hipStream_t segs[STREAM_COUNT];
+  for (int i = 0; i < gpuInfo->nstream && i < STREAM_COUNT; ++i)
+    if (gpuInfo->stream && gpuInfo->stream[i]) segs[nseg++] = reinterpret_cast<hipStream_t>(gpuInfo->stream[i]);
+  if (nseg == 0) segs[nseg++] = st->stream;
+  nseg = (int)std::min<long long>(nseg, (nvis + kRows - 1) / kRows);
+  // the frequency lines (and any spectral table) were queued on the
+  // library's stream: the segments start after them
+  hipEvent_t ready;
+  OBIT_HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  OBIT_HIP_CHECK(hipEventRecord(ready, st->stream));
+  const long long per = (nvis + nseg - 1) / nseg;
+  for (int i = 0; i < nseg; ++i) {
+    const long long r0 = i * per, n = std::min(per, nvis - r0);
+    if (n <= 0) break;
+    hipStream_t s = segs[i];
+    OBIT_HIP_CHECK(hipStreamWaitEvent(s, ready, 0));
+    float *dv = st->stage + r0 * gpuInfo->lenvis;
+    OBIT_HIP_CHECK(hipMemcpyAsync(dv, gpuInfo->h_data + r0 * gpuInfo->lenvis, (size_t)n * rowBytes,
+                                  hipMemcpyHostToDevice, s));
+    Desc ds = d;
+    ds.vis = dv;
+    ds.nrow = n;
+    ds.nrowblk = (int)((n + kRows - 1) / kRows);
+    ds.nitems = (long long)ds.nrowblk * ds.nIFsel * ds.nchunk;
+    rc = launch_split(st, ds, cpt, tile, st->wgPerCU, s);
+    if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
+    OBIT_HIP_CHECK(hipMemcpyAsync(gpuInfo->h_data + r0 * gpuInfo->lenvis, dv, (size_t)n * rowBytes,
+                                  hipMemcpyDeviceToHost, s));
+  }
+  for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipStreamSynchronize(segs[i]));
+  OBIT_HIP_CHECK(hipEventDestroy(ready));
 }
 
 void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
@@ -1120,7 +1184,7 @@ int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo) {
 int ObitHIPSkyModelDFTSetVariant(GPUInfo *gpuInfo, int cpt, int tile, int wgPerCU) {
   State *st = find_state(gpuInfo);
   if (!st) return obit::fail(obit::kState, "DFTSetVariant before DFTInit");
-  if (cpt != 16 && cpt != 32) return obit::fail(obit::kInvalid, "cpt must be 16 or 32");
+  if (cpt != 8 && cpt != 16 && cpt != 32) return obit::fail(obit::kInvalid, "cpt must be 8, 16 or 32");
   if (tile != 32 && tile != 64 && tile != 128) return obit::fail(obit::kInvalid, "tile must be 32, 64 or 128");
   if (wgPerCU < 1 || wgPerCU > 8) return obit::fail(obit::kInvalid, "wgPerCU must be 1..8");
   st->cpt = cpt; st->tile = tile; st->wgPerCU = wgPerCU;

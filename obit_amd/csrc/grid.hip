@@ -649,25 +649,39 @@ __global__ void k_trim_centre(const FacetDev *facets, int fi, int nplane, float 
   }
 }
 
-// ObitFArray2DCenter's quadrant swap of the C2R output (in-place layout,
-// rows of 2*(nx/2+1) floats), then ObitFArrayMulColRow's correction as the
-// reference applies it to its square images: pixel (x, y) times
-// xCorr[y]/norm and yCorr[x] (src/ObitFArray.c:3116-3123).
-__global__ void k_center_correct(const float *fft, int nx, int ny, int nplane, const float *xc,
-                                 const float *yc, const float *invNorm, float *img) {
-  const long long total = (long long)nplane * ny * nx;
-  const int pitch = 2 * (nx / 2 + 1);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int x = (int)(i % nx);
-    const long long yp = i / nx;
-    const int y = (int)(yp % ny);
-    const int p = (int)(yp / ny);
-    const int xs = (x + nx / 2) % nx, ys = (y + ny / 2) % ny;
-    const float v = fft[((size_t)p * ny + ys) * pitch + xs];
-    const float row = xc[y] * invNorm[p];
-    img[i] = v * row * yc[x];
-  }
+// ObitFArray2DCenter's quadrant swap and ObitFArrayMulColRow's correction
+// (src/ObitFArray.c:3116-3123) of a C2R of the planes as they lie in the
+// gridder (v halves not swapped): a cyclic shift of the v axis by ny/2 before the transform
+// is the factor (-1)^y after it, so pixel (x, y) of the centred image is
+// (-1)^ys out[ys][xs] with xs = (x + nx/2) % nx, ys = (y + ny/2) % ny,
+// times xCorr[y]/norm and yCorr[x].  One thread per 4 pixels of a row
+// (nx a multiple of 8, so the 4 source pixels are contiguous).
+__global__ void k_center_correct4(const float *fft, int nx, int ny, const float *xc, const float *yc,
+                                  const float *invNorm, float *img) {
+  const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = blockIdx.y, p = blockIdx.z;
+  if (x0 >= nx) return;
+  int ys = y + ny / 2;
+  if (ys >= ny) ys -= ny;
+  int xs = x0 + nx / 2;
+  if (xs >= nx) xs -= nx;
+  const float4 v = *reinterpret_cast<const float4 *>(fft + ((size_t)p * ny + ys) * nx + xs);
+  const float4 c = *reinterpret_cast<const float4 *>(yc + x0);
+  const float r = ((ys & 1) ? -xc[y] : xc[y]) * invNorm[p];
+  *reinterpret_cast<float4 *>(img + ((size_t)p * ny + y) * nx + x0) =
+      make_float4(v.x * r * c.x, v.y * r * c.y, v.z * r * c.z, v.w * r * c.w);
+}
+
+// Beam normalisation of every plane on the device: the centre pixel of the
+// centred image, (-1)^0 out[p][0][0] x xCorr[ny/2] x yCorr[nx/2]; norm and
+// its reciprocal (as the reference forms it, 1 / max(1e-20, norm)).
+__global__ void k_beam_norm(const float *fft, int nx, int ny, int nplane, const float *xc, const float *yc,
+                            float *norm, float *invNorm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nplane) return;
+  const float v = fft[(size_t)p * ny * nx] * xc[ny / 2] * yc[nx / 2];
+  norm[p] = v;
+  invNorm[p] = (float)(1.0 / fmax(1.0e-20, (double)v));
 }
 
 // Per-sample outputs of prep_sample for the parity tests.
@@ -723,7 +737,8 @@ struct GpuState {
   unsigned long long *d_walk = nullptr;  // staged samples, base-cell moves of the last pass
   // FFT to images: C2R plan for (nx, ny, nplane), complex scratch, corrections
   hipfftHandle fftPlan = 0;
-  int fftNx = 0, fftNy = 0, fftNplane = 0;
+  int fftNx = 0, fftNy = 0, fftNplane = 0, fftPitch = 0, fftDist = 0;
+  float *h_norm = nullptr;  // pinned: plane norms / reciprocals staged for the caller
   float *d_fft = nullptr;
   size_t fftCap = 0;
   float *d_corr = nullptr;  // xCorr[nx], yCorr[ny], 1/norm[nplane]
@@ -1303,6 +1318,7 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     if (gs->h_tot) (void)hipHostFree(gs->h_tot);
     if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
     (void)hipFree(gs->d_fft); (void)hipFree(gs->d_corr);
+    if (gs->h_norm) (void)hipHostFree(gs->h_norm);
     if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
     (void)hipStreamDestroy(gs->stream);
     obit::grid::GridCtx *c = obit::grid::get_ctx(cudaInfo, false);
@@ -1407,89 +1423,91 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
   const auto &f = gs->h_facets[k];
   const int nx = f.nx, ny = f.ny, nplane = in->h_gridInfo->nplane;
   const GridInfo *gi = in->h_gridInfo;
-  if (nx != ny || (nx & 1))
-    return obit::fail(obit::kInvalid, "FFT2Image needs a square, even facet (got %d x %d): the "
-                      "reference's correction indexes rows and columns alike", nx, ny);
+  if (nx != ny || (nx % 8))
+    return obit::fail(obit::kInvalid, "FFT2Image needs a square facet with a side a multiple of 8 (got %d x %d): "
+                      "the reference's correction indexes rows and columns alike", nx, ny);
   if (!gi->h_convfn) return obit::fail(obit::kState, "no convolving function");
-  // trim and centre the (flipped) planes into the complex scratch
-  const size_t nxo = (size_t)nx / 2 + 1;
-  const size_t need = (size_t)nplane * ny * nxo * 2;
+  if (nplane > 1024) return obit::fail(obit::kInvalid, "%d planes: at most 1024", nplane);
+  // real output of the C2R for every plane
+  const size_t need = (size_t)nplane * ny * nx;
   if (need > gs->fftCap) {
     (void)hipFree(gs->d_fft);
     gs->d_fft = nullptr;
+    gs->fftCap = 0;
     OBIT_HIP_TRY(hipMalloc(&gs->d_fft, sizeof(float) * need));
     gs->fftCap = need;
   }
-  {
-    const long long total = (long long)nplane * ny * nxo;
-    const int nb = (int)std::min<long long>((total + 255) / 256, (long long)gs->nCU * 16);
-    hipLaunchKernelGGL(obit::grid::k_trim_centre, dim3(std::max(nb, 1)), dim3(256), 0, gs->stream,
-                       gs->d_facets, k, nplane, gs->d_fft);
-    OBIT_HIP_TRY(hipGetLastError());
-  }
-  // one batched in-place C2R (FFTW's unnormalised backward transform)
-  if (!gs->fftPlan || gs->fftNx != nx || gs->fftNy != ny || gs->fftNplane != nplane) {
+  // one batched C2R (FFTW's unnormalised backward transform) read straight
+  // from the gridder's padded half planes: row pitch lrow/2 complex, the
+  // u = 0 column hw complex into each row, planes sizeGrid floats apart
+  const int pitchC = f.lrow / 2;
+  if (!gs->fftPlan || gs->fftNx != nx || gs->fftNy != ny || gs->fftNplane != nplane ||
+      gs->fftPitch != pitchC || gs->fftDist != f.sizeGrid) {
     if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
     gs->fftPlan = 0;
-    int n[2] = {ny, nx}, inembed[2] = {ny, (int)nxo}, onembed[2] = {ny, 2 * (int)nxo};
-    hipfftResult r = hipfftPlanMany(&gs->fftPlan, 2, n, inembed, 1, ny * (int)nxo, onembed, 1,
-                                    ny * 2 * (int)nxo, HIPFFT_C2R, nplane);
+    int n[2] = {ny, nx}, inembed[2] = {ny, pitchC}, onembed[2] = {ny, nx};
+    hipfftResult r = hipfftPlanMany(&gs->fftPlan, 2, n, inembed, 1, f.sizeGrid / 2, onembed, 1, ny * nx,
+                                    HIPFFT_C2R, nplane);
     if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftPlanMany: %d", (int)r);
     gs->fftNx = nx;
     gs->fftNy = ny;
     gs->fftNplane = nplane;
+    gs->fftPitch = pitchC;
+    gs->fftDist = f.sizeGrid;
   }
   if (hipfftSetStream(gs->fftPlan, gs->stream) != HIPFFT_SUCCESS)
     return obit::fail(obit::kHipError, "hipfftSetStream");
   {
-    hipfftResult r = hipfftExecC2R(gs->fftPlan, (hipfftComplex *)gs->d_fft, (hipfftReal *)gs->d_fft);
+    hipfftComplex *src = reinterpret_cast<hipfftComplex *>(f.grid + 2 * f.hw);
+    hipfftResult r = hipfftExecC2R(gs->fftPlan, src, reinterpret_cast<hipfftReal *>(gs->d_fft));
     if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftExecC2R: %d", (int)r);
   }
-  // gridding corrections (GridCorrFn, computed once per size) and normalisation
-  if (gs->corrNx != nx || gs->corrNy != ny) {
+  // gridding corrections (GridCorrFn, computed once per size): xCorr[nx],
+  // yCorr[ny], then the planes' norms and reciprocals
+  const size_t ncorr = (size_t)nx + ny + 2 * 1024;
+  if (gs->corrNx != nx || gs->corrNy != ny || gs->corrCap < ncorr) {
     gs->h_corr.assign((size_t)nx + ny, 0.0f);
     ObitHostGridCorrFn(nx, nx / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell, gs->h_corr.data());
     ObitHostGridCorrFn(ny, ny / 2 + 1, gi->h_convfn, gi->convWidth, gi->convNperCell,
                        gs->h_corr.data() + nx);
+    if (gs->corrCap < ncorr) {
+      (void)hipFree(gs->d_corr);
+      gs->d_corr = nullptr;
+      gs->corrCap = 0;
+      OBIT_HIP_TRY(hipMalloc(&gs->d_corr, sizeof(float) * ncorr));
+      gs->corrCap = ncorr;
+    }
+    OBIT_HIP_TRY(hipMemcpy(gs->d_corr, gs->h_corr.data(), sizeof(float) * (nx + ny), hipMemcpyHostToDevice));
     gs->corrNx = nx;
     gs->corrNy = ny;
-    gs->corrCap = 0;  // upload again below
   }
-  std::vector<float> corr(gs->h_corr);
-  corr.resize((size_t)nx + ny + nplane);
+  if (!gs->h_norm) OBIT_HIP_TRY(hipHostMalloc(&gs->h_norm, sizeof(float) * 2 * 1024, hipHostMallocDefault));
+  float *d_norm = gs->d_corr + nx + ny, *d_inv = d_norm + 1024;
+  const float *xc = gs->d_corr, *yc = gs->d_corr + nx;
   if (isBeam) {
-    // centre pixel of each plane after the swap = element (0, 0) before it
-    std::vector<float> c0(nplane);
-    OBIT_HIP_TRY(hipMemcpy2DAsync(c0.data(), sizeof(float), gs->d_fft, sizeof(float) * ny * 2 * nxo,
-                                  sizeof(float), nplane, hipMemcpyDeviceToHost, gs->stream));
-    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+    hipLaunchKernelGGL(obit::grid::k_beam_norm, dim3((nplane + 63) / 64), dim3(64), 0, gs->stream, gs->d_fft, nx,
+                       ny, nplane, xc, yc, d_norm, d_inv);
+    OBIT_HIP_TRY(hipGetLastError());
+  } else {
+    OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));  // the pinned staging is free
+    for (int p = 0; p < nplane; ++p) gs->h_norm[p] = (float)(1.0 / std::max(1.0e-20, (double)beamNorm[p]));
+    OBIT_HIP_TRY(hipMemcpyAsync(d_inv, gs->h_norm, sizeof(float) * nplane, hipMemcpyHostToDevice, gs->stream));
+  }
+  {
+    dim3 grid((unsigned)((nx / 4 + 255) / 256), (unsigned)ny, (unsigned)nplane);
+    hipLaunchKernelGGL(obit::grid::k_center_correct4, grid, dim3(256), 0, gs->stream, gs->d_fft, nx, ny, xc, yc,
+                       d_inv, d_image);
+    OBIT_HIP_TRY(hipGetLastError());
+  }
+  if (isBeam)
+    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_norm, d_norm, sizeof(float) * nplane, hipMemcpyDeviceToHost, gs->stream));
+  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
+  if (isBeam) {
     for (int p = 0; p < nplane; ++p) {
-      beamNorm[p] = c0[p] * corr[nx / 2] * corr[nx + ny / 2];
+      beamNorm[p] = gs->h_norm[p];
       if (beamNorm[p] == 0.0f) return obit::fail(obit::kInvalid, "beam peak is zero in plane %d", p);
     }
   }
-  for (int p = 0; p < nplane; ++p)
-    corr[(size_t)nx + ny + p] = (float)(1.0 / std::max(1.0e-20, (double)beamNorm[p]));
-  if (corr.size() > gs->corrCap) {
-    (void)hipFree(gs->d_corr);
-    gs->d_corr = nullptr;
-    OBIT_HIP_TRY(hipMalloc(&gs->d_corr, sizeof(float) * corr.size()));
-    gs->corrCap = corr.size();
-    OBIT_HIP_TRY(hipMemcpyAsync(gs->d_corr, corr.data(), sizeof(float) * (nx + ny),
-                                hipMemcpyHostToDevice, gs->stream));
-  }
-  OBIT_HIP_TRY(hipMemcpyAsync(gs->d_corr + nx + ny, corr.data() + nx + ny, sizeof(float) * nplane,
-                              hipMemcpyHostToDevice, gs->stream));
-  {
-    const long long total = (long long)nplane * ny * nx;
-    const int nb = (int)std::min<long long>((total + 255) / 256, (long long)gs->nCU * 16);
-    hipLaunchKernelGGL(obit::grid::k_center_correct, dim3(std::max(nb, 1)), dim3(256), 0, gs->stream,
-                       gs->d_fft, nx, ny, nplane, gs->d_corr, gs->d_corr + nx,
-                       gs->d_corr + nx + ny, d_image);
-    OBIT_HIP_TRY(hipGetLastError());
-  }
-  // corr is read by the async copy above
-  OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
   return obit::kOK;
 }
 

@@ -17,6 +17,7 @@
 #include "ObitHIPVis.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -193,6 +194,42 @@ int ObitHostDFTSetMod(GPUInfo *g, const ObitHostSkyModel *sm, const float *comps
 /* ObitGPUSkyModelDFTCalc (src/ObitGPUSkyModel.c:500-520), applied to a
  * buffer of any length in nVisPIO-row pieces, as ObitSkyModelSubUV reads
  * and writes one uv buffer at a time (src/ObitSkyModel.c:1000-1026). */
+/* memcpy of a large buffer in slices on several threads: one thread's
+ * copy into pinned memory runs well below the host's memory bandwidth,
+ * and the per-buffer path copies every row in and out of the pinned
+ * buffer. */
+typedef struct { char *dst; const char *src; size_t n; } copy_slice;
+static void *copy_run(void *a)
+{
+  copy_slice *c = (copy_slice *)a;
+  memcpy(c->dst, c->src, c->n);
+  return NULL;
+}
+static void par_memcpy(void *dst, const void *src, size_t n)
+{
+  enum { kMaxThreads = 8 };
+  const size_t minSlice = (size_t)4 << 20;
+  int nt = (int)(n / minSlice);
+  if (nt > kMaxThreads) nt = kMaxThreads;
+  if (nt < 2) { memcpy(dst, src, n); return; }
+  pthread_t th[kMaxThreads];
+  copy_slice cs[kMaxThreads];
+  const size_t per = (n / nt + 63) & ~(size_t)63;
+  size_t off = 0;
+  int started = 0;
+  for (int i = 0; i < nt && off < n; i++) {
+    cs[i].dst = (char *)dst + off;
+    cs[i].src = (const char *)src + off;
+    cs[i].n = (n - off < per) ? n - off : per;
+    off += cs[i].n;
+    if (i == nt - 1 || off >= n) { copy_run(&cs[i]); break; }  /* the last slice on this thread */
+    if (pthread_create(&th[i], NULL, copy_run, &cs[i]) != 0) { copy_run(&cs[i]); continue; }
+    started |= 1 << i;
+  }
+  for (int i = 0; i < nt; i++)
+    if (started & (1 << i)) pthread_join(th[i], NULL);
+}
+
 int ObitHostDFTCalc(GPUInfo *g, float *buffer, long long nvis)
 {
   long long done = 0;
@@ -205,9 +242,9 @@ int ObitHostDFTCalc(GPUInfo *g, float *buffer, long long nvis)
     size_t bytes = (size_t)n * g->lenvis * sizeof(float);
     g->nvis = n;
     g->h_visInfo->nvis = n;
-    memcpy(g->h_data, buffer + done * g->lenvis, bytes);
+    par_memcpy(g->h_data, buffer + done * g->lenvis, bytes);
     ObitCUDASkyModelDFTCalc(g);
-    memcpy(buffer + done * g->lenvis, g->h_data, bytes);
+    par_memcpy(buffer + done * g->lenvis, g->h_data, bytes);
     done += n;
   }
   return OBIT_HIP_OK;

@@ -165,16 +165,17 @@ def test_fft2image_matches_oracle(oracle, uv4):
     corr = np.zeros(n, np.float32)
     _lib.load().ObitHostGridCorrFn(n, n // 2 + 1, cf.ctypes.data_as(C.POINTER(C.c_float)), 7, 100,
                                    corr.ctypes.data_as(C.POINTER(C.c_float)))
+    # the transform consumes the grid planes: take them first
     with _gpu_grid(uv4, f, beam=True) as gg:
         gg.grid(uv4.buffer)
         gg.flip()
-        beam, norm = gg.fft_image(0, True)
         planes_b = gg.to_host(0)
+        beam, norm = gg.fft_image(0, True)
     with _gpu_grid(uv4, f) as gg:
         gg.grid(uv4.buffer)
         gg.flip()
-        img, _ = gg.fft_image(0, False, norm)
         planes_i = gg.to_host(0)
+        img, _ = gg.fft_image(0, False, norm)
     rb, rnorm = oracle.fft2image(planes_b, cf, True, corr=(corr, corr))
     ri, _ = oracle.fft2image(planes_i, cf, False, rnorm, corr=(corr, corr))
     assert abs(norm[0] / rnorm[0] - 1) < 1e-5
