@@ -50,6 +50,69 @@ class BeamImageDesc(C.Structure):
                 ("cdelt", C.c_float * 2)]
 
 
+class GPUBeamVisInfo(C.Structure):
+    """GPUBeamVisInfo (include/ObitHIPBeamModel.h)."""
+    _fields_ = [(n, C.c_int) for n in (
+        "nvis", "nprod", "nrparm", "lenvis", "nchan", "ilocu", "ilocv", "ilocw", "iloct", "ilocb",
+        "ilocsu", "chanb", "chane", "incf", "nIF", "nSpec", "IFb", "IFe", "incif", "nstok", "stokb",
+        "stoke", "incs", "stype", "kincif", "kincf")] + [
+        ("h_freqRat", C.c_void_p), ("d_freqRat", C.c_void_p), ("d_freqScale", C.c_void_p)]
+
+
+class GPUBeamModelInfo(C.Structure):
+    """GPUBeamModelInfo (include/ObitHIPBeamModel.h)."""
+    _fields_ = [(n, C.c_int) for n in ("nterm", "nSpec", "size", "doSwap", "type", "opType", "stokType")] + [
+        ("h_specIndex", C.c_void_p), ("d_specIndex", C.c_void_p), ("nmodel", C.c_int),
+        ("d_model", C.c_void_p), ("stokFact", C.c_float * 4)]
+
+
+class CUDAFArray(C.Structure):
+    """CUDAFArray (include/ObitHIPBeamModel.h)."""
+    _fields_ = [("h_array", C.c_void_p), ("d_array", C.c_void_p), ("d_FArray", C.c_void_p),
+                ("fblank", C.c_float), ("ndim", C.c_int), ("naxis", C.c_int * 5)]
+
+
+class CUDAImageDesc(C.Structure):
+    """Leading members of CUDAImageDesc (include/ObitHIPBeamModel.h)."""
+    _fields_ = ([("bitpix", C.c_long), ("naxis", C.c_long), ("inaxes", C.c_long * 7),
+                 ("ctype", (C.c_char * 21) * 7)]
+                + [(n, C.c_float * 7) for n in ("cdelt", "crpix", "crota")]
+                + [(n, C.c_float) for n in ("altCrpix", "xshift", "yshift", "beamMaj", "beamMin", "beamPA")]
+                + [(n, C.c_char * 41) for n in ("object", "teles", "instrument", "observer", "obsdat", "date",
+                                                 "origin", "bunit")]
+                + [("epoch", C.c_float), ("equinox", C.c_float), ("obsra", C.c_double),
+                   ("obsdec", C.c_double), ("crval", C.c_double * 7)])
+
+
+class CUDABeamInterp(C.Structure):
+    """CUDABeamInterp (include/ObitHIPBeamModel.h)."""
+    _fields_ = [("d_BeamInterp", C.c_void_p), ("nx", C.c_int), ("ny", C.c_int), ("hwidth", C.c_int),
+                ("denom", C.c_float * 10), ("h_myArray", C.c_void_p), ("d_myArray", C.c_void_p),
+                ("d_myDesc", C.c_void_p)]
+
+
+class GPUBeamAntInfo(C.Structure):
+    """GPUBeamAntInfo (include/ObitHIPBeamModel.h)."""
+    _fields_ = [(n, C.c_int) for n in ("numAnt", "numAntType", "numJChan", "ilocb", "iloca1", "iloca2",
+                                       "ilocsa")] + [
+        (n, C.c_float) for n in ("s2p", "c2p", "refantSize", "refSizeFact")] + [
+        (n, C.c_void_p) for n in ("h_BeamFreq", "d_BeamFreq", "h_AntType", "d_AntType", "h_visJChann",
+                                  "d_visJChann", "d_Jones", "d_BeamInterp")]
+
+
+class GPUBeamInfo(C.Structure):
+    """GPUBeamInfo (include/ObitHIPBeamModel.h)."""
+    _fields_ = [(n, C.c_int) for n in ("nvis", "nchan", "nrparm", "lenvis", "modelSize", "nstream", "prtLv",
+                                       "oldnVisPIO", "cuda_device", "bufferSize", "nmodel")] + [
+        ("timeRange", C.c_float * 2), ("delta_ParAng", C.c_float), ("curParAng", C.c_float),
+        ("visRange", C.c_long * 2), ("doUnit", C.c_int), ("haveUnit", C.c_int),
+        ("d_model", C.c_void_p), ("d_freq", C.c_void_p), ("h_data", C.c_void_p), ("d_data", C.c_void_p),
+        ("d_visInfo", C.c_void_p), ("h_visInfo", C.POINTER(GPUBeamVisInfo)),
+        ("d_modelInfo", C.c_void_p), ("h_modelInfo", C.POINTER(GPUBeamModelInfo)),
+        ("d_antInfo", C.c_void_p), ("h_antInfo", C.POINTER(GPUBeamAntInfo)),
+        ("stream", C.c_void_p), ("cycleDone", C.c_void_p)]
+
+
 class HostSkyModel(C.Structure):
     _fields_ = [
         ("modType", C.c_int), ("nSpecTerm", C.c_int),
@@ -177,11 +240,17 @@ _PROTOS = {
     "ObitHIPBeamDFTSetBeams": (_I, [_P, _P, _P, _P, _F, _F]),
     "ObitHIPBeamDFTJonesFromBeams": (_I, [_P, _P, _F]),
     "ObitHIPBeamDFTDestroy": (_I, [_P]),
+    # Obit's beam-model entry points (ObitHIPBeamModel.h)
+    "ObitCUDABeamModelDFTInit": (None, [C.POINTER(GPUBeamInfo)]),
+    "ObitCUDABeamModelDFTSetMod": (None, [C.POINTER(GPUBeamInfo)]),
+    "ObitCUDABeamModelDFTCalc": (None, [C.POINTER(GPUBeamInfo), _I, _I, _F, C.POINTER(C.c_long)]),
+    "ObitCUDABeamModelDFTShutdown": (None, [C.POINTER(GPUBeamInfo)]),
     # host classes (csrc/host.h)
     "ObitHostDFTCreate": (_P, []),
     "ObitHostDFTInit": (_I, [_P, C.POINTER(HostUVDesc), C.POINTER(HostSkyModel), _I]),
     "ObitHostDFTSetMod": (_I, [_P, C.POINTER(HostSkyModel), _P, _I, _I]),
     "ObitHostDFTCalc": (_I, [_P, _P, _LL]),
+    "ObitHostDFTCalcTimed": (_I, [_P, _P, _LL, _DP]),
     "ObitHostDFTShutdown": (None, [_P]),
     "ObitHostConvFuncSph": (None, [_P]),
     "ObitHostGridCreate": (_P, [_I, _I, _IP, C.POINTER(HostUVDesc), _I, _I, _I]),

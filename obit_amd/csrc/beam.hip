@@ -25,10 +25,16 @@
 // reads its product from the table in global memory.
 #include "common.h"
 #include "ObitHIPBeam.h"
+#include "ObitHIPBeamModel.h"
+#include "ObitHIPVis.h"
 #include "phasor.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace obit {
@@ -582,6 +588,70 @@ void release(ObitHIPBeam *h) {
   delete h;
 }
 
+// no component with flux: the data are left as they are (the reference
+// kernels' gotSome)
+bool beam_active(const ObitHIPBeam *h) {
+  return h->desc.ncomp > 0 && (h->d_tab ? h->anyTab : h->anyFlux);
+}
+
+// The coherency products of the current Jones table at parallactic angle
+// parAng (k_beam_coh), on stream.
+int beam_coh(ObitHIPBeam *h, float parAng, hipStream_t stream) {
+  using namespace obit;
+  const ObitHIPBeamDesc &D = h->desc;
+  const double c2 = std::cos(2.0 * (double)parAng), s2 = std::sin(2.0 * (double)parAng);
+  const long long ncoh = (long long)D.ntype * D.ntype * D.ncomp * D.njchan;
+  hipLaunchKernelGGL(beam::k_beam_coh, dim3((unsigned)((ncoh + 255) / 256)), dim3(256), 0, stream,
+                     h->d_jones, h->d_coh, D.ntype, D.ncomp, D.njchan, D.stokType, (float)c2, (float)s2);
+  OBIT_HIP_TRY(hipGetLastError());
+  return kOK;
+}
+
+// The model applied to nrow rows at d_vis from the coherency products
+// (k_beam_dft), on stream.
+int beam_dft(ObitHIPBeam *h, float *d_vis, long long nrow, hipStream_t stream) {
+  using namespace obit;
+  const ObitHIPBeamDesc &D = h->desc;
+  beam::Desc d;
+  d.vis = d_vis;
+  d.nrow = nrow;
+  d.lrec = D.lrec; d.nrparm = D.nrparm;
+  d.ilocu = D.ilocu; d.ilocv = D.ilocv; d.ilocw = D.ilocw;
+  d.ilocb = D.ilocb; d.iloca1 = D.iloca1; d.iloca2 = D.iloca2;
+  d.nif = D.nif; d.nchan = D.nchan; d.incf = D.incf; d.incif = D.incif;
+  const int cpt = D.ncorr == 2 ? beam::kCPT2 : beam::kCPT4;
+  d.nchunk = (D.nchan + cpt - 1) / cpt;
+  const long long nrowblk = (nrow + beam::kRows - 1) / beam::kRows;
+  if (nrowblk > 0x7fffffffLL) return fail(kInvalid, "ObitHIPBeamDFTCalc: too many rows");
+  d.nrowblk = (int)nrowblk;
+  d.nitems = nrowblk * D.nif * d.nchunk;
+  d.ncomp = D.ncomp; d.ntype = D.ntype; d.nant = D.nant; d.njchan = D.njchan;
+  d.npair = D.ntype * D.ntype;
+  d.fIF = h->d_freq; d.dfIF = h->d_freq + D.nif;
+  d.cx = h->d_pos; d.cy = h->d_pos + D.ncomp; d.cz = h->d_pos + 2 * (size_t)D.ncomp;
+  d.camp = h->d_amp; d.cg = h->d_g;
+  d.antType = h->d_antType; d.jmap = h->d_jmap;
+  d.coh = h->d_coh;
+  d.tab = h->d_tab;
+  d.op = D.opType;
+  for (int i = 0; i < 4; ++i) d.sf[i] = D.stokFact[i];
+  d.chanb = D.chanb; d.chane = D.chane; d.IFb = D.IFb; d.IFe = D.IFe;
+  long long grid = (long long)std::max(h->ncu, 1) * beam::kBlocksPerCU;
+  if (grid > d.nitems) grid = d.nitems;
+  void *args[] = {&d};
+  const void *fn;
+  const bool sp = h->d_tab != nullptr;
+  if (D.ncorr == 2) {
+    if (h->gauss) fn = sp ? (const void *)beam::k_beam_dft<2, beam::kCPT2, true, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, true, false>;
+    else fn = sp ? (const void *)beam::k_beam_dft<2, beam::kCPT2, false, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, false, false>;
+  } else {
+    if (h->gauss) fn = sp ? (const void *)beam::k_beam_dft<4, beam::kCPT4, true, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, true, false>;
+    else fn = sp ? (const void *)beam::k_beam_dft<4, beam::kCPT4, false, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, false, false>;
+  }
+  OBIT_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(beam::kRows), args, 0, stream));
+  return kOK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -704,53 +774,10 @@ int ObitHIPBeamDFTCalc(ObitHIPBeam *h, float *d_vis, long long nrow, float parAn
   using namespace obit;
   if (!h || (!d_vis && nrow > 0) || nrow < 0) return fail(kInvalid, "ObitHIPBeamDFTCalc: bad arguments");
   if (!h->haveMod) return fail(kState, "ObitHIPBeamDFTCalc: no component table (ObitHIPBeamDFTSetMod)");
-  const ObitHIPBeamDesc &D = h->desc;
-  // no component with flux: the data are left as they are (the reference
-  // kernels' gotSome)
-  if (nrow == 0 || D.ncomp == 0 || !(h->d_tab ? h->anyTab : h->anyFlux)) return kOK;
-  const double c2 = std::cos(2.0 * (double)parAng), s2 = std::sin(2.0 * (double)parAng);
-  const long long ncoh = (long long)D.ntype * D.ntype * D.ncomp * D.njchan;
-  hipLaunchKernelGGL(beam::k_beam_coh, dim3((unsigned)((ncoh + 255) / 256)), dim3(256), 0, h->stream,
-                     h->d_jones, h->d_coh, D.ntype, D.ncomp, D.njchan, D.stokType, (float)c2, (float)s2);
-  OBIT_HIP_TRY(hipGetLastError());
-  beam::Desc d;
-  d.vis = d_vis;
-  d.nrow = nrow;
-  d.lrec = D.lrec; d.nrparm = D.nrparm;
-  d.ilocu = D.ilocu; d.ilocv = D.ilocv; d.ilocw = D.ilocw;
-  d.ilocb = D.ilocb; d.iloca1 = D.iloca1; d.iloca2 = D.iloca2;
-  d.nif = D.nif; d.nchan = D.nchan; d.incf = D.incf; d.incif = D.incif;
-  const int cpt = D.ncorr == 2 ? beam::kCPT2 : beam::kCPT4;
-  d.nchunk = (D.nchan + cpt - 1) / cpt;
-  const long long nrowblk = (nrow + beam::kRows - 1) / beam::kRows;
-  if (nrowblk > 0x7fffffffLL) return fail(kInvalid, "ObitHIPBeamDFTCalc: too many rows");
-  d.nrowblk = (int)nrowblk;
-  d.nitems = nrowblk * D.nif * d.nchunk;
-  d.ncomp = D.ncomp; d.ntype = D.ntype; d.nant = D.nant; d.njchan = D.njchan;
-  d.npair = D.ntype * D.ntype;
-  d.fIF = h->d_freq; d.dfIF = h->d_freq + D.nif;
-  d.cx = h->d_pos; d.cy = h->d_pos + D.ncomp; d.cz = h->d_pos + 2 * (size_t)D.ncomp;
-  d.camp = h->d_amp; d.cg = h->d_g;
-  d.antType = h->d_antType; d.jmap = h->d_jmap;
-  d.coh = h->d_coh;
-  d.tab = h->d_tab;
-  d.op = D.opType;
-  for (int i = 0; i < 4; ++i) d.sf[i] = D.stokFact[i];
-  d.chanb = D.chanb; d.chane = D.chane; d.IFb = D.IFb; d.IFe = D.IFe;
-  long long grid = (long long)std::max(h->ncu, 1) * beam::kBlocksPerCU;
-  if (grid > d.nitems) grid = d.nitems;
-  void *args[] = {&d};
-  const void *fn;
-  const bool sp = h->d_tab != nullptr;
-  if (D.ncorr == 2) {
-    if (h->gauss) fn = sp ? (const void *)beam::k_beam_dft<2, beam::kCPT2, true, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, true, false>;
-    else fn = sp ? (const void *)beam::k_beam_dft<2, beam::kCPT2, false, true> : (const void *)beam::k_beam_dft<2, beam::kCPT2, false, false>;
-  } else {
-    if (h->gauss) fn = sp ? (const void *)beam::k_beam_dft<4, beam::kCPT4, true, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, true, false>;
-    else fn = sp ? (const void *)beam::k_beam_dft<4, beam::kCPT4, false, true> : (const void *)beam::k_beam_dft<4, beam::kCPT4, false, false>;
-  }
-  OBIT_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(beam::kRows), args, 0, h->stream));
-  return kOK;
+  if (!beam_active(h) || nrow == 0) return kOK;
+  int rc = beam_coh(h, parAng, h->stream);
+  if (!rc) rc = beam_dft(h, d_vis, nrow, h->stream);
+  return rc;
 }
 
 int ObitHIPBeamDFTSetAmps(ObitHIPBeam *h, const float *amp) {
@@ -850,6 +877,344 @@ int ObitHIPBeamDFTDestroy(ObitHIPBeam *h) {
   else if (h) (void)hipDeviceSynchronize();
   release(h);
   return obit::kOK;
+}
+
+}  // extern "C"
+
+// ---------------- Obit's beam-model entry points (ObitHIPBeamModel.h) ----------------
+//
+// ObitCUDABeamModelDFTInit / SetMod / Calc / Shutdown over the handle
+// above: the GPUBeamInfo the reference's host class fills
+// (src/ObitGPUBeamModel.c:212-896) is read into a descriptor, the
+// component rows into the handle's tables, and Calc streams the caller's
+// pinned rows through its d_data[] buffers on its streams.
+
+namespace {
+
+struct BeamAbi {
+  ObitHIPBeam *h = nullptr;
+  ObitHIPBeamDesc desc{};
+  std::vector<double> fIF, dfIF;
+  std::vector<int> antType, visJChan;
+  std::vector<float> fscale;    // d_freqScale as the caller holds it
+  int kincf = 1, kincif = 1;
+  std::vector<float> pos;       // [ncomp][2] beam offsets dX, dY (deg)
+  const void *beamsFrom = nullptr;  // d_BeamInterp the beam images were read from
+  hipEvent_t cohDone = nullptr;
+};
+
+std::mutex g_abiMu;
+std::unordered_map<const GPUBeamInfo *, BeamAbi *> g_abi;
+
+// the reference's checkCudaErrors reports and exits with EXIT_FAILURE
+[[noreturn]] void abi_exit(const char *where) {
+  std::fprintf(stderr, "obit_hip: %s: %s\n", where, ObitHIPLastError());
+  std::fflush(stderr);
+  std::exit(EXIT_FAILURE);
+}
+
+BeamAbi *abi_find(const GPUBeamInfo *g) {
+  std::lock_guard<std::mutex> lk(g_abiMu);
+  auto it = g_abi.find(g);
+  return it == g_abi.end() ? nullptr : it->second;
+}
+
+template <class T>
+int d2h(std::vector<T> &out, const void *src, size_t n, const char *what) {
+  using namespace obit;
+  if (!src && n) return fail(kInvalid, "%s: null device pointer", what);
+  out.resize(n);
+  if (n) OBIT_HIP_TRY(hipMemcpy(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  return kOK;
+}
+
+// The record layout, frequency lines and antenna tables of gpuInfo.
+int abi_init(const GPUBeamInfo *g, BeamAbi *st) {
+  using namespace obit;
+  if (!g || !g->h_visInfo || !g->h_antInfo) return fail(kInvalid, "null gpuInfo, h_visInfo or h_antInfo");
+  const GPUBeamVisInfo &V = *g->h_visInfo;
+  const GPUBeamAntInfo &A = *g->h_antInfo;
+  const int nif = std::max(V.nIF, 1);
+  if (V.nchan < 1 || V.nchan % nif) return fail(kInvalid, "nchan %d is not a multiple of nIF %d", V.nchan, nif);
+  const int nch = V.nchan / nif;
+  if (V.incs != 3) return fail(kInvalid, "Stokes increment %d (3 expected)", V.incs);
+  if (V.nstok < 2) return fail(kInvalid, "%d Stokes (XX, YY at least expected)", V.nstok);
+  st->kincf = V.kincf;
+  st->kincif = V.kincif;
+  auto fidx = [&](int c, int i) { return (long long)c * V.kincf + (long long)i * V.kincif; };
+  if (fidx(nch - 1, nif - 1) >= (long long)nch * nif || fidx(nch - 1, 0) < 0 || fidx(0, nif - 1) < 0)
+    return fail(kInvalid, "frequency table increments %d, %d outside %d x %d", V.kincf, V.kincif, nch, nif);
+  int rc = d2h(st->fscale, V.d_freqScale, (size_t)nch * nif, "d_freqScale");
+  if (rc) return rc;
+  // one line per IF through its first and last channel
+  st->fIF.assign(nif, 0.0);
+  st->dfIF.assign(nif, 0.0);
+  for (int i = 0; i < nif; ++i) {
+    const double f0 = st->fscale[fidx(0, i)], f1 = st->fscale[fidx(nch - 1, i)];
+    st->fIF[i] = f0;
+    st->dfIF[i] = nch > 1 ? (f1 - f0) / (nch - 1) : 0.0;
+    for (int c = 0; c < nch; ++c) {
+      const double f = st->fscale[fidx(c, i)], lin = f0 + c * st->dfIF[i];
+      if (!(std::fabs(f - lin) <= 1e-6 * std::fabs(lin)))
+        return fail(kInvalid, "channel %d of IF %d: frequency factor %.9g is %.3g off the IF's line", c, i, f,
+                    f - lin);
+    }
+  }
+  rc = d2h(st->antType, A.d_AntType, (size_t)std::max(A.numAnt, 0), "d_AntType");
+  if (!rc) rc = d2h(st->visJChan, A.d_visJChann, (size_t)nch * nif, "d_visJChann");
+  if (rc) return rc;
+  ObitHIPBeamDesc &d = st->desc;
+  d = ObitHIPBeamDesc{};
+  d.lrec = V.lenvis; d.nrparm = V.nrparm;
+  d.ilocu = V.ilocu; d.ilocv = V.ilocv; d.ilocw = V.ilocw;
+  d.ilocb = A.ilocb; d.iloca1 = A.iloca1; d.iloca2 = A.iloca2;
+  d.nif = nif; d.nchan = nch; d.incf = V.incf; d.incif = V.incif;
+  d.ncorr = V.nstok >= 4 ? 4 : 2;
+  d.ntype = A.numAntType; d.nant = A.numAnt; d.njchan = A.numJChan;
+  d.chanb = V.chanb; d.chane = V.chane; d.IFb = V.IFb; d.IFe = V.IFe;
+  return kOK;
+}
+
+// The component rows of h_modelInfo into a new handle.
+int abi_setmod(const GPUBeamInfo *g, BeamAbi *st) {
+  using namespace obit;
+  if (!g->h_modelInfo) return fail(kInvalid, "null h_modelInfo");
+  const GPUBeamModelInfo &M = *g->h_modelInfo;
+  const GPUBeamVisInfo &V = *g->h_visInfo;
+  const bool gauss = M.type == GPUBeamModTypeGaussTSpec;
+  if (M.type != GPUBeamModTypePointTSpec && !gauss)
+    return fail(kInvalid, "model type %d not supported (PointTSpec, GaussTSpec: the rows ObitSkyModelVMBeamMF loads)",
+                (int)M.type);
+  const int base = gauss ? 10 : 7;
+  const int n = M.nmodel, L = M.size, nspec = M.nSpec;
+  if (n < 0 || nspec < 1 || L < base + 2 * nspec)
+    return fail(kInvalid, "%d components of %d floats with %d subbands (%d floats needed)", n, L, nspec,
+                base + 2 * nspec);
+  if (M.stokType < 1 || M.stokType > 4 || M.opType < 0 || M.opType > 2)
+    return fail(kInvalid, "stokType %d / opType %d", M.stokType, (int)M.opType);
+  const int nif = st->desc.nif, nch = st->desc.nchan;
+  std::vector<float> rows, frat;
+  std::vector<int> spec;
+  int rc = d2h(rows, M.d_model, (size_t)n * L, "d_model");
+  if (!rc) rc = d2h(spec, M.d_specIndex, (size_t)nch * nif, "d_specIndex");
+  if (!rc) rc = d2h(frat, V.d_freqRat, (size_t)nspec, "d_freqRat");
+  if (rc) return rc;
+  for (size_t f = 0; f < spec.size(); ++f)
+    if (spec[f] < 0 || spec[f] >= nspec)
+      return fail(kInvalid, "subband %d of frequency entry %zu outside 0..%d", spec[f], f, nspec - 1);
+  const int clen = gauss ? 7 : 4;
+  const size_t nc = (size_t)std::max(n, 1);
+  std::vector<float> comps(nc * clen, 0.0f), amps(nc * nif * nch, 0.0f);
+  st->pos.assign(2 * nc, 0.0f);
+  for (int k = 0; k < n; ++k) {
+    const float *r = rows.data() + (size_t)k * L;
+    float *c = comps.data() + (size_t)k * clen;
+    c[0] = 1.0f;  // the amplitude is the per-channel table below
+    c[1] = r[4]; c[2] = r[5]; c[3] = r[6];
+    if (gauss) { c[4] = r[7]; c[5] = r[8]; c[6] = r[9]; }
+    st->pos[2 * k] = r[2];
+    st->pos[2 * k + 1] = r[3];
+    for (int i = 0; i < nif; ++i)
+      for (int ch = 0; ch < nch; ++ch) {
+        const long long f = (long long)ch * st->kincf + (long long)i * st->kincif;
+        const int s = spec[f];
+        const double flux = r[base + 2 * s], si = r[base + 2 * s + 1];
+        amps[((size_t)k * nif + i) * nch + ch] =
+            flux == 0.0 ? 0.0f : (float)(flux * std::pow((double)st->fscale[f] * (double)frat[s], si));
+      }
+  }
+  ObitHIPBeamDesc d = st->desc;
+  d.ncomp = n; d.compLen = clen;
+  d.stokType = M.stokType; d.opType = (int)M.opType;
+  for (int i = 0; i < 4; ++i) d.stokFact[i] = M.stokFact[i];
+  if (st->h) {
+    (void)ObitHIPBeamDFTDestroy(st->h);
+    st->h = nullptr;
+  }
+  st->beamsFrom = nullptr;
+  hipStream_t s0 = (g->nstream > 0 && g->stream) ? (hipStream_t)g->stream[0] : nullptr;
+  rc = ObitHIPBeamDFTCreate(&st->h, &d, st->fIF.data(), st->dfIF.data(), st->antType.data(),
+                            st->visJChan.data(), (void *)s0);
+  if (!rc) rc = ObitHIPBeamDFTSetMod(st->h, comps.data());
+  if (!rc) rc = ObitHIPBeamDFTSetAmps(st->h, amps.data());
+  if (!rc) st->desc = d;
+  return rc;
+}
+
+// Beam images and geometry from the caller's interpolators
+// (CUDABeamInterp objects made by ObitGPUCUDABeamInterpCreate,
+// src/ObitGPUBeamInterp.c:232-295): 8 per antenna type, the first one's
+// image descriptor placing the components.
+int abi_beams(const GPUBeamInfo *g, BeamAbi *st) {
+  using namespace obit;
+  const GPUBeamAntInfo &A = *g->h_antInfo;
+  const int nt = st->desc.ntype, nch = st->desc.njchan;
+  if (!A.d_BeamInterp) return fail(kState, "no beam interpolators (d_BeamInterp)");
+  std::vector<CUDABeamInterp *> ptrs;
+  int rc = d2h(ptrs, A.d_BeamInterp, (size_t)nt * 8, "d_BeamInterp");
+  if (rc) return rc;
+  std::vector<ObitHIPBeamImageDesc> desc(nt);
+  std::vector<float> images;
+  float fblank = 0.0f;
+  static const char *kProj[8] = {"-SIN", "-TAN", "-ARC", "-NCP", "-GLS", "-MER", "-AIT", "-STG"};
+  for (int t = 0; t < nt; ++t) {
+    for (int k = 0; k < 8; ++k) {
+      std::vector<CUDABeamInterp> bi;
+      std::vector<CUDAFArray> fa;
+      rc = d2h(bi, ptrs[t * 8 + k], 1, "beam interpolator");
+      if (!rc) rc = d2h(fa, bi[0].d_myArray, 1, "beam array");
+      if (rc) return rc;
+      const int nx = fa[0].naxis[0], ny = fa[0].ndim > 1 ? fa[0].naxis[1] : 1;
+      const int nz = fa[0].ndim > 2 ? fa[0].naxis[2] : 1;
+      if (k == 0) {
+        std::vector<CUDAImageDesc> id;
+        rc = d2h(id, bi[0].d_myDesc, 1, "beam image descriptor");
+        if (rc) return rc;
+        ObitHIPBeamImageDesc &b = desc[t];
+        b.nx = nx; b.ny = ny; b.nchan = nz;
+        b.hwidth = bi[0].hwidth;
+        b.proj = 0;
+        for (int j = 0; j < 8; ++j)
+          if (std::equal(kProj[j], kProj[j] + 4, &id[0].ctype[0][4])) b.proj = j + 1;
+        if (b.proj > 3) return fail(kInvalid, "type %d beam projection %s not supported", t, kProj[b.proj - 1]);
+        b.crval[0] = id[0].crval[0]; b.crval[1] = id[0].crval[1];
+        b.crpix[0] = id[0].crpix[0]; b.crpix[1] = id[0].crpix[1];
+        b.cdelt[0] = id[0].cdelt[0]; b.cdelt[1] = id[0].cdelt[1];
+        if (nz != nch) return fail(kInvalid, "type %d beam has %d planes, numJChan %d", t, nz, nch);
+        if (t == 0) {
+          fblank = fa[0].fblank;
+          images.reserve((size_t)nt * 8 * nch * nx * ny);
+        }
+      } else if (nx != desc[t].nx || ny != desc[t].ny || nz != nch) {
+        return fail(kInvalid, "type %d beam image %d is %d x %d x %d, image 0 %d x %d x %d", t, k, nx, ny, nz,
+                    desc[t].nx, desc[t].ny, nch);
+      }
+      std::vector<float> pix;
+      rc = d2h(pix, fa[0].d_array, (size_t)nx * ny * nz, "beam pixels");
+      if (rc) return rc;
+      images.insert(images.end(), pix.begin(), pix.end());
+    }
+  }
+  std::vector<float> bfreq;
+  rc = d2h(bfreq, A.d_BeamFreq, (size_t)nch, "d_BeamFreq");
+  if (!rc) rc = ObitHIPBeamDFTSetBeams(st->h, desc.data(), images.data(), bfreq.data(), A.refSizeFact, fblank);
+  if (!rc) st->beamsFrom = A.d_BeamInterp;
+  return rc;
+}
+
+int abi_calc(const GPUBeamInfo *g, BeamAbi *st, int doJones, int doUnit, float parAng, const long visRange[2]) {
+  using namespace obit;
+  ObitHIPBeam *h = st->h;
+  if (!h) return fail(kState, "no model (ObitCUDABeamModelDFTSetMod)");
+  if (!visRange) return fail(kInvalid, "null visRange");
+  if (g->nstream < 1 || !g->stream || !g->d_data) return fail(kInvalid, "no streams or device buffers");
+  const GPUBeamAntInfo &A = *g->h_antInfo;
+  hipStream_t s0 = (hipStream_t)g->stream[0];
+  if (h->stream != s0) {
+    OBIT_HIP_TRY(hipStreamSynchronize(h->stream));
+    h->stream = s0;
+  }
+  const size_t njones = (size_t)st->desc.ntype * std::max(st->desc.ncomp, 1) * st->desc.njchan * 8 * sizeof(float);
+  int rc = kOK;
+  if (doJones) {
+    if (A.d_BeamInterp != st->beamsFrom) rc = abi_beams(g, st);
+    if (!rc) rc = ObitHIPBeamDFTJonesFromBeams(h, st->pos.data(), parAng);
+    if (!rc && A.d_Jones) OBIT_HIP_TRY(hipMemcpyAsync(A.d_Jones, h->d_jones, njones, hipMemcpyDeviceToDevice, s0));
+  } else if (doUnit) {
+    rc = ObitHIPBeamDFTSetJones(h, nullptr);
+    if (!rc && A.d_Jones) OBIT_HIP_TRY(hipMemcpyAsync(A.d_Jones, h->d_jones, njones, hipMemcpyDeviceToDevice, s0));
+  } else if (A.d_Jones) {
+    // the table as the caller's d_Jones holds it
+    OBIT_HIP_TRY(hipMemcpyAsync(h->d_jones, A.d_Jones, njones, hipMemcpyDeviceToDevice, s0));
+  }
+  if (rc) return rc;
+  const long long r0 = visRange[0], r1 = visRange[1];
+  if (r1 < r0 || !beam_active(h)) return kOK;
+  if (r0 < 0) return fail(kInvalid, "visRange %lld..%lld", r0, r1);
+  if (!g->h_data) return fail(kInvalid, "null h_data");
+  const long long lenvis = st->desc.lrec;
+  // rows of one d_data[] buffer ((nvis + 5) lenvis floats over nstream, as
+  // the host class allocates them)
+  const long long cap = (long long)(((size_t)(g->nvis + 5) * lenvis * sizeof(float) / g->nstream) /
+                                    ((size_t)lenvis * sizeof(float)));
+  if (cap < 1) return fail(kInvalid, "device buffers hold no row (nvis %d, %d streams)", g->nvis, g->nstream);
+  rc = beam_coh(h, parAng, s0);
+  if (rc) return rc;
+  if (!st->cohDone) OBIT_HIP_TRY(hipEventCreateWithFlags(&st->cohDone, hipEventDisableTiming));
+  OBIT_HIP_TRY(hipEventRecord(st->cohDone, s0));
+  const int ns = g->nstream;
+  int used = 0;
+  for (long long a = r0, k = 0; a <= r1; a += cap, ++k) {
+    const long long nrow = std::min(cap, r1 - a + 1);
+    const int si = (int)(k % ns);
+    hipStream_t s = (hipStream_t)g->stream[si];
+    float *dv = g->d_data[si];
+    float *hv = g->h_data + a * lenvis;
+    const size_t bytes = (size_t)nrow * lenvis * sizeof(float);
+    if (!dv) return fail(kInvalid, "d_data[%d] is null", si);
+    OBIT_HIP_TRY(hipStreamWaitEvent(s, st->cohDone, 0));
+    OBIT_HIP_TRY(hipMemcpyAsync(dv, hv, bytes, hipMemcpyHostToDevice, s));
+    rc = beam_dft(h, dv, nrow, s);
+    if (rc) return rc;
+    OBIT_HIP_TRY(hipMemcpyAsync(hv, dv, bytes, hipMemcpyDeviceToHost, s));
+    used = std::max(used, si + 1);
+  }
+  for (int i = 0; i < used; ++i) OBIT_HIP_TRY(hipStreamSynchronize((hipStream_t)g->stream[i]));
+  return kOK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ObitCUDABeamModelDFTInit(GPUBeamInfo *gpuInfo) {
+  BeamAbi *st = abi_find(gpuInfo);
+  const bool fresh = st == nullptr;
+  if (fresh) st = new BeamAbi();
+  if (abi_init(gpuInfo, st)) {
+    if (fresh) delete st;
+    abi_exit("ObitCUDABeamModelDFTInit");
+  }
+  if (fresh) {
+    std::lock_guard<std::mutex> lk(g_abiMu);
+    g_abi[gpuInfo] = st;
+  }
+}
+
+void ObitCUDABeamModelDFTSetMod(GPUBeamInfo *gpuInfo) {
+  BeamAbi *st = abi_find(gpuInfo);
+  if (!st) {
+    obit::fail(obit::kState, "no layout (ObitCUDABeamModelDFTInit)");
+    abi_exit("ObitCUDABeamModelDFTSetMod");
+  }
+  if (abi_setmod(gpuInfo, st)) abi_exit("ObitCUDABeamModelDFTSetMod");
+}
+
+void ObitCUDABeamModelDFTCalc(GPUBeamInfo *gpuInfo, int doJones, int doUnit, float parAng, long visRange[2]) {
+  BeamAbi *st = abi_find(gpuInfo);
+  if (!st) {
+    obit::fail(obit::kState, "no layout (ObitCUDABeamModelDFTInit)");
+    abi_exit("ObitCUDABeamModelDFTCalc");
+  }
+  if (abi_calc(gpuInfo, st, doJones, doUnit, parAng, visRange)) abi_exit("ObitCUDABeamModelDFTCalc");
+}
+
+void ObitCUDABeamModelDFTShutdown(GPUBeamInfo *gpuInfo) {
+  BeamAbi *st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_abiMu);
+    auto it = g_abi.find(gpuInfo);
+    if (it != g_abi.end()) {
+      st = it->second;
+      g_abi.erase(it);
+    }
+  }
+  if (st) {
+    if (st->h) (void)ObitHIPBeamDFTDestroy(st->h);
+    if (st->cohDone) (void)hipEventDestroy(st->cohDone);
+    delete st;
+  }
+  (void)hipDeviceSynchronize();
 }
 
 }  // extern "C"

@@ -114,3 +114,43 @@ def test_dft_path_query_without_state():
     p = C.c_int(-1)
     assert lib.ObitHIPSkyModelDFTGetPath(C.c_void_p(777), C.byref(p)) == 4
     assert p.value == -1
+
+
+def test_beam_model_struct_layouts(tmp_path):
+    """The ctypes mirrors of the beam-model structures have the sizes and
+    member offsets include/ObitHIPBeamModel.h gives them under the host C
+    compiler (the layout the reference's host class fills)."""
+    from obit_amd import _lib
+    names = ["GPUBeamVisInfo", "GPUBeamModelInfo", "CUDAFArray", "CUDAImageDesc", "CUDABeamInterp",
+             "GPUBeamAntInfo", "GPUBeamInfo"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ObitHIPBeamModel.h"', 'int main(void) {']
+    want = []
+    for n in names:
+        st = getattr(_lib, n)
+        lines.append(f'  printf("%zu\\n", sizeof({n}));')
+        want.append(C.sizeof(st))
+        for f in st._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({n}, {f[0]}));')
+            want.append(getattr(st, f[0]).offset)
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == want
+
+
+def test_beam_model_entry_points_need_init():
+    """Calc on a GPUBeamInfo that Init never saw exits with EXIT_FAILURE and
+    a message (as the reference's checkCudaErrors does); run in a child, with
+    no GPU."""
+    code = ("import ctypes as C\n"
+            "from obit_amd import _lib\n"
+            "lib = _lib.load()\n"
+            "g = _lib.GPUBeamInfo()\n"
+            "vr = (C.c_long * 2)(0, 9)\n"
+            "lib.ObitCUDABeamModelDFTCalc(C.byref(g), 0, 1, 0.0, vr)\n")
+    r = subprocess.run(["python3", "-c", code], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "ObitCUDABeamModelDFTCalc" in r.stderr and "Init" in r.stderr
