@@ -168,21 +168,37 @@ def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
     """Obit's per-buffer path: ObitCUDASkyModelDFTCalc on host buffers of
     nVisPIO rows (copy in to pinned memory, H2D, kernel, D2H, copy out),
     over the first `rows` rows: the rate a caller that hands over host
-    buffers sees, PCIe included."""
+    buffers sees, PCIe included.  The host class alternates two pinned
+    buffers so one piece's host copies overlap the next piece's device
+    work; the shares reported are the time inside ObitCUDASkyModelDFTCalc
+    (H2D, kernels, D2H on the caller's streams), the host copies' own time,
+    and the split kernel alone on one resident nVisPIO buffer."""
     from obit_amd.skymodel import SkyModelDFT
     sub = uv.subset(np.arange(min(rows, uv.nrow)))
+    ev = sub.nrow * sub.nif * sub.nchan * sky.comps.shape[0]
     with SkyModelDFT(sub, sky, op="sub", nVisPIO=nvispio) as sm:
         if variant:
             sm.set_variant(*variant)
         sm.calc(sub.buffer[:nvispio].copy())  # warm
         buf = sub.buffer.copy()
+        times = np.zeros(3)
+        sm.calc(buf, times)
+        dt = times[0]
+    # the kernel alone on one resident buffer of nVisPIO rows
+    one = sub.subset(np.arange(nvispio))
+    with SkyModelDFT(one, sky, op="sub") as sk:
+        sk.set_vis(one.buffer)
+        sk.calc_resident()
+        lib.ObitCUDADeviceSynchronize(None)
         t0 = time.perf_counter()
-        sm.calc(buf)
-        dt = time.perf_counter() - t0
-    ev = sub.nrow * sub.nif * sub.nchan * sky.comps.shape[0]
-    # the host class's own share, bounded: the same bytes copied into and
-    # out of a pinned buffer by one thread (ObitGPUSkyModelDFTCalc's memcpy;
-    # the host class here splits each copy over up to 8 threads)
+        for _ in range(3):
+            sk.calc_resident()
+        lib.ObitCUDADeviceSynchronize(None)
+        kern = (time.perf_counter() - t0) / 3
+    npieces = (sub.nrow + nvispio - 1) // nvispio
+    # the reference host class's own share, bounded: the same bytes copied
+    # into and out of a pinned buffer by one thread (ObitGPUSkyModelDFTCalc's
+    # memcpy)
     chunk = nvispio * sub.lrec * 4
     pinned = lib.ObitCUDAUtilAllocHost(chunk)
     try:
@@ -197,9 +213,13 @@ def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
     finally:
         lib.ObitCUDAUtilFreeHost(pinned)
     log(f"per-buffer path (host buffers, PCIe included): {sub.nrow} rows in {dt:.3f}s -> "
-        f"{ev / dt / 1e12:.3f} Tevals/s; the same copies on one thread take {dt_mem:.3f}s")
+        f"{ev / dt / 1e12:.3f} Tevals/s ({100 * kern * npieces / dt:.0f}% of the kernel alone); per {nvispio}-row piece: "
+        f"{dt / npieces * 1e3:.2f} ms wall, {times[1] / npieces * 1e3:.2f} ms in DFTCalc, "
+        f"{times[2] / npieces * 1e3:.2f} ms host copies (overlapped), kernel alone {kern * 1e3:.2f} ms; "
+        f"one-thread copies {dt_mem / npieces * 1e3:.2f} ms")
     return {"pcie_inclusive_evals_per_s": ev / dt, "pcie_rows": sub.nrow, "nVisPIO": nvispio,
-            "pcie_s": dt, "host_memcpy_one_thread_s": dt_mem}
+            "pcie_s": dt, "pieces": npieces, "dftcalc_s": times[1], "host_copies_s": times[2],
+            "kernel_per_piece_s": kern, "host_memcpy_one_thread_s": dt_mem}
 
 
 def grid_accuracy(uv, facets, b, e, rows=2000, nfacet=2):
@@ -755,6 +775,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid-only", action="store_true", help="time the gridding workload alone")
     ap.add_argument("--beam-only", action="store_true", help="time the Jones-corrected DFT alone")
+    ap.add_argument("--pcie-only", action="store_true", help="time the per-buffer DFT path alone")
     ap.add_argument("--workload5", action="store_true",
                     help="run workload 5 at full size (89 GB resident; about two minutes) and exit")
     args = ap.parse_args()
@@ -779,6 +800,12 @@ def main():
         uv, sky, _ = synth.workload(2)
         print(json.dumps({"beam_dft": bench_beam(lib, uv, sky, check=not args.no_cpu),
                           "beam_dft_4corr": bench_beam4(lib, uv, sky)}), flush=True)
+        return
+    if args.pcie_only:
+        from obit_amd import synth
+        uv, sky, _ = synth.workload(2)
+        vt = tuple(int(x) for x in args.variant.split(",")) if args.variant else None
+        print(json.dumps({"pcie": bench_pcie(lib, uv, sky, vt)}), flush=True)
         return
     if args.grid_only:
         grid, _ = bench_grid(lib, args.grid_steps)

@@ -147,7 +147,14 @@ class HostFieldGeom(C.Structure):
         ("imCrval", C.c_double * 2), ("imCrota", C.c_float), ("do3D", C.c_int),
         ("xPxOff", C.c_float), ("yPxOff", C.c_float),
         ("uvCrval", C.c_double * 2), ("uvCrota", C.c_float),
+        ("imCrpix", C.c_float * 2), ("imCdelt", C.c_float * 2), ("imNaxis", C.c_int * 2),
+        ("xshift", C.c_float), ("yshift", C.c_float),
     ]
+
+
+class HostLoadOpts(C.Structure):
+    _fields_ = [("modType", C.c_int), ("nSpecTerm", C.c_int), ("factor", C.c_float),
+                ("minFlux", C.c_float), ("minDFT", C.c_float), ("noNeg", C.c_int)]
 
 
 _P = C.c_void_p
@@ -262,6 +269,9 @@ _PROTOS = {
     "ObitHostPlanesFromBins": (None, [_I, _I, _IP, _IP, _I, _IP]),
     "ObitHostGridCorrFn": (None, [_I, _I, _FP, _I, _I, _FP]),
     "ObitHostLoadCompsPoint": (_I, [C.POINTER(HostFieldGeom), _FP, _I, _I, _F, _FP]),
+    "ObitHostCCMergeWidth": (_I, [_I, _I]),
+    "ObitHostCCMergeSel": (_I, [_P, _I, _I, _I, _P, _I, _I, _P, _P]),
+    "ObitHostLoadComps": (_I, [C.POINTER(HostFieldGeom), _P, _I, _I, _I, _P, C.POINTER(HostLoadOpts), _P, _I]),
 }
 
 _lib = None

@@ -58,6 +58,62 @@ def load_comps_point(cc, im_crval, im_crota, uv_crval, uv_crota=0.0, factor=1.0)
     return out
 
 
+def cc_merge(rows, have_z: bool = False, no_parms: int = 0, status=None, start: int = 1, end: int | None = None):
+    """ObitTableCCUtilMergeSel on CC rows [Flux, DeltaX, DeltaY, (DeltaZ),
+    parms[no_parms]] (ObitHostCCMergeSel): merged rows [flux, X, Y, (Z),
+    spectral terms] by decreasing |flux|, and the first selected row's
+    parms[0..3]."""
+    lib = _lib.load()
+    rows = np.ascontiguousarray(rows, dtype=np.float32)
+    nrow = rows.shape[0]
+    if rows.ndim != 2 or rows.shape[1] != 3 + int(have_z) + no_parms:
+        raise ValueError("rows must be [nrow, 3 + haveZ + no_parms]")
+    width = lib.ObitHostCCMergeWidth(int(have_z), no_parms)
+    out = np.zeros((max(nrow, 1), width), np.float32)
+    parms = np.zeros(4, np.float32)
+    st = None if status is None else np.ascontiguousarray(status, dtype=np.int32)
+    n = lib.ObitHostCCMergeSel(_lib.ptr(rows), nrow, int(have_z), no_parms,
+                               None if st is None else _lib.ptr(st), start,
+                               nrow if end is None else end, _lib.ptr(out), _lib.ptr(parms))
+    if n < 0:
+        raise ValueError("ObitHostCCMergeSel: bad input")
+    return out[:n].copy(), parms
+
+
+def load_comps(geom: dict, merged, parms, have_z: bool = False, nterms: int = 0, mod_type: int = 0,
+               n_spec_term: int = 0, factor: float = 1.0, min_flux: float = 0.0, min_dft: float = 0.0,
+               no_neg: bool = False) -> np.ndarray:
+    """ObitSkyModelLoadComps for one field (ObitHostLoadComps) from merged
+    CC rows (cc_merge); geom holds the field's and the uv data's geometry
+    (ObitHostFieldGeom members: im_crval, im_crota, do3D, xPxOff, yPxOff,
+    uv_crval, uv_crota, im_crpix, im_cdelt, im_naxis, xshift, yshift)."""
+    lib = _lib.load()
+    g = _lib.HostFieldGeom()
+    g.imCrval[0], g.imCrval[1] = geom["im_crval"]
+    g.imCrota = geom["im_crota"]
+    g.do3D = int(geom["do3D"])
+    g.xPxOff, g.yPxOff = geom["xPxOff"], geom["yPxOff"]
+    g.uvCrval[0], g.uvCrval[1] = geom["uv_crval"]
+    g.uvCrota = geom["uv_crota"]
+    g.imCrpix[0], g.imCrpix[1] = geom["im_crpix"]
+    g.imCdelt[0], g.imCdelt[1] = geom["im_cdelt"]
+    g.imNaxis[0], g.imNaxis[1] = geom["im_naxis"]
+    g.xshift, g.yshift = geom["xshift"], geom["yshift"]
+    o = _lib.HostLoadOpts()
+    o.modType, o.nSpecTerm, o.factor = mod_type, n_spec_term, factor
+    o.minFlux, o.minDFT, o.noNeg = min_flux, min_dft, int(no_neg)
+    merged = np.ascontiguousarray(merged, dtype=np.float32)
+    parms = np.ascontiguousarray(parms, dtype=np.float32)
+    nm = merged.shape[0]
+    width = 4 + (3 if mod_type in (1, 11) else 0) + (n_spec_term if mod_type >= 10 else 0)
+    out = np.zeros((max(nm, 1), width), np.float32)
+    n = lib.ObitHostLoadComps(C.byref(g), _lib.ptr(merged), nm, int(have_z), nterms, _lib.ptr(parms),
+                              C.byref(o), _lib.ptr(out), nm)
+    if n < 0:
+        raise ValueError("ObitHostLoadComps: bad input or a combination of model types Obit refuses")
+    return out[:n].copy()
+
+
 class SkyModelDFT:
     """A component model applied to visibilities by the HIP DFT.
 
@@ -115,11 +171,19 @@ class SkyModelDFT:
                                               comps.shape[1], comps.shape[0]), "DFTSetMod")
 
     # -- per-buffer path (Obit's DFTCalc) --
-    def calc(self, buffer: np.ndarray) -> np.ndarray:
-        """Apply the model in place to a host buffer, nVisPIO rows per call."""
+    def calc(self, buffer: np.ndarray, times: np.ndarray | None = None) -> np.ndarray:
+        """Apply the model in place to a host buffer, nVisPIO rows per call.
+        times (float64[3]), when given, accumulates the wall time, the time
+        inside ObitCUDASkyModelDFTCalc and the host copies' own time."""
         if buffer.dtype != np.float32 or not buffer.flags["C_CONTIGUOUS"]:
             raise ValueError("buffer must be C-contiguous float32")
-        _lib.check(self.lib.ObitHostDFTCalc(self.g, _lib.ptr(buffer), buffer.shape[0]), "DFTCalc")
+        if times is None:
+            _lib.check(self.lib.ObitHostDFTCalc(self.g, _lib.ptr(buffer), buffer.shape[0]), "DFTCalc")
+        else:
+            if times.dtype != np.float64 or times.shape != (3,):
+                raise ValueError("times must be float64[3]")
+            _lib.check(self.lib.ObitHostDFTCalcTimed(self.g, _lib.ptr(buffer), buffer.shape[0],
+                                                     times.ctypes.data_as(C.POINTER(C.c_double))), "DFTCalc")
         return buffer
 
     # -- resident path --
