@@ -184,17 +184,22 @@ def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
         times = np.zeros(3)
         sm.calc(buf, times)
         dt = times[0]
-    # the kernel alone on one resident buffer of nVisPIO rows
+    # the kernel alone on one resident buffer of nVisPIO rows, for each
+    # channels-per-thread choice (what the per-buffer path picks from)
     one = sub.subset(np.arange(nvispio))
+    by_cpt = {}
     with SkyModelDFT(one, sky, op="sub") as sk:
         sk.set_vis(one.buffer)
-        sk.calc_resident()
-        lib.ObitCUDADeviceSynchronize(None)
-        t0 = time.perf_counter()
-        for _ in range(3):
+        for c in (32, 16, 8):
+            sk.set_variant(c, 64, 4)
             sk.calc_resident()
-        lib.ObitCUDADeviceSynchronize(None)
-        kern = (time.perf_counter() - t0) / 3
+            lib.ObitCUDADeviceSynchronize(None)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                sk.calc_resident()
+            lib.ObitCUDADeviceSynchronize(None)
+            by_cpt[c] = (time.perf_counter() - t0) / 3
+    kern = min(by_cpt.values())
     npieces = (sub.nrow + nvispio - 1) // nvispio
     # the reference host class's own share, bounded: the same bytes copied
     # into and out of a pinned buffer by one thread (ObitGPUSkyModelDFTCalc's
@@ -215,11 +220,12 @@ def bench_pcie(lib, uv, sky, variant, rows=100000, nvispio=10000):
     log(f"per-buffer path (host buffers, PCIe included): {sub.nrow} rows in {dt:.3f}s -> "
         f"{ev / dt / 1e12:.3f} Tevals/s ({100 * kern * npieces / dt:.0f}% of the kernel alone); per {nvispio}-row piece: "
         f"{dt / npieces * 1e3:.2f} ms wall, {times[1] / npieces * 1e3:.2f} ms in DFTCalc, "
-        f"{times[2] / npieces * 1e3:.2f} ms host copies (overlapped), kernel alone {kern * 1e3:.2f} ms; "
+        f"{times[2] / npieces * 1e3:.2f} ms host copies (overlapped), kernel alone "
+        f"{', '.join(f'{v * 1e3:.2f} ms at {c} ch' for c, v in by_cpt.items())}; "
         f"one-thread copies {dt_mem / npieces * 1e3:.2f} ms")
     return {"pcie_inclusive_evals_per_s": ev / dt, "pcie_rows": sub.nrow, "nVisPIO": nvispio,
             "pcie_s": dt, "pieces": npieces, "dftcalc_s": times[1], "host_copies_s": times[2],
-            "kernel_per_piece_s": kern, "host_memcpy_one_thread_s": dt_mem}
+            "kernel_per_piece_s": kern, "kernel_per_piece_s_by_cpt": by_cpt, "host_memcpy_one_thread_s": dt_mem}
 
 
 def grid_accuracy(uv, facets, b, e, rows=2000, nfacet=2):

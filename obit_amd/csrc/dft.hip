@@ -928,21 +928,22 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
     OBIT_HIP_CHECK(hipMalloc(&st->stage, bytes));
     st->stageCap = bytes;
   }
-  // channels per thread for this buffer: a persistent launch takes
-  // ceil(items / slots) rounds of items that each cost about cpt / e(cpt)
-  // (e = 1, 0.9, 0.6 for 32, 16, 8 channels: the per-chunk phasor set-up
-  // weighs more on short chunks); a 10,000-row buffer of 4 IFs x 64
-  // channels has 320 items at 32 channels (one round on a third of the
-  // GPU), 640 at 16 (one round), 1,280 at 8 (two rounds)
+  // channels per thread for this buffer: the kernel is bound by the VALU
+  // of each CU, so a launch lasts about as long as its busiest CU's share,
+  // ceil(items / CUs) items that each cost about cpt / e(cpt) (e = 1, 0.9,
+  // 0.6 for 32, 16, 8 channels: the per-chunk phasor set-up weighs more on
+  // short chunks); a 10,000-row buffer of 4 IFs x 64 channels has 320
+  // items at 32 channels (2 on the busiest CU: 64), 640 at 16 (3: 53), 1,280
+  // at 8 (5: 67)
   Desc d;
   int rc = make_desc(st, gpuInfo, st->stage, nvis, 32, d);
   if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
-  const long long slots = (long long)st->nCU * st->wgPerCU;
+  const long long ncu = std::max(st->nCU, 1);
   int cpt = 32, tile = st->tile;
   double best = 0.0;
   for (int c : {32, 16, 8}) {
     const long long items = (long long)d.nrowblk * d.nIFsel * ((d.nchsel + c - 1) / c);
-    const double cost = (double)((items + slots - 1) / slots) * c / (c == 32 ? 1.0 : c == 16 ? 0.9 : 0.6);
+    const double cost = (double)((items + ncu - 1) / ncu) * c / (c == 32 ? 1.0 : c == 16 ? 0.9 : 0.6);
     if (c == 32 || cost < best) { best = cost; cpt = c; }
   }
   if (cpt == 8) tile = 64;
@@ -957,30 +958,39 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
     return;
   }
   st->lastPath = 0;
-  // the buffer in up to STREAM_COUNT row segments on the caller's streams
+  // the buffer in up to 3 row segments on the caller's streams
   // (ObitGPUSkyModelDFTInit creates them): each segment's upload, launch
-  // and download queue on its own stream, so copies overlap the other
-  // segments' launches (as the reference's ObitCUDASkyModel.cu:804-963)
+  // and download queue on its own stream, so one segment's kernel runs
+  // while the next is uploaded and the previous downloaded (as the
+  // reference's ObitCUDASkyModel.cu:804-963 pipelines its streams).  The
+  // uploads are chained (each waits for the one before) so the first
+  // segment's rows arrive at the full link rate; a fourth stream would
+  // share a hardware queue with another (4 queues per process, one taken
+  // by the library's own stream) and run behind it.
   int nseg = 0;
-  hipStream_t segs[STREAM_COUNT];
-  for (int i = 0; i < gpuInfo->nstream && i < STREAM_COUNT; ++i)
+  hipStream_t segs[3];
+  for (int i = 0; i < gpuInfo->nstream && i < STREAM_COUNT && nseg < 3; ++i)
     if (gpuInfo->stream && gpuInfo->stream[i]) segs[nseg++] = reinterpret_cast<hipStream_t>(gpuInfo->stream[i]);
   if (nseg == 0) segs[nseg++] = st->stream;
-  nseg = (int)std::min<long long>(nseg, (nvis + kRows - 1) / kRows);
+  // whole row blocks per segment, every segment non-empty
+  nseg = (int)std::min<long long>(nseg, d.nrowblk);
+  const long long blkPer = (d.nrowblk + nseg - 1) / nseg;
+  nseg = (int)((d.nrowblk + blkPer - 1) / blkPer);
   // the frequency lines (and any spectral table) were queued on the
   // library's stream: the segments start after them
-  hipEvent_t ready;
+  hipEvent_t ready, up[3];
   OBIT_HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipEventCreateWithFlags(&up[i], hipEventDisableTiming));
   OBIT_HIP_CHECK(hipEventRecord(ready, st->stream));
-  const long long per = (nvis + nseg - 1) / nseg;
   for (int i = 0; i < nseg; ++i) {
-    const long long r0 = i * per, n = std::min(per, nvis - r0);
-    if (n <= 0) break;
+    const long long r0 = i * blkPer * kRows;
+    const long long n = std::min(blkPer * kRows, nvis - r0);
     hipStream_t s = segs[i];
-    OBIT_HIP_CHECK(hipStreamWaitEvent(s, ready, 0));
+    OBIT_HIP_CHECK(hipStreamWaitEvent(s, i ? up[i - 1] : ready, 0));
     float *dv = st->stage + r0 * gpuInfo->lenvis;
     OBIT_HIP_CHECK(hipMemcpyAsync(dv, gpuInfo->h_data + r0 * gpuInfo->lenvis, (size_t)n * rowBytes,
                                   hipMemcpyHostToDevice, s));
+    OBIT_HIP_CHECK(hipEventRecord(up[i], s));
     Desc ds = d;
     ds.vis = dv;
     ds.nrow = n;
@@ -993,6 +1003,7 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
   }
   for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipStreamSynchronize(segs[i]));
   OBIT_HIP_CHECK(hipEventDestroy(ready));
+  for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipEventDestroy(up[i]));
 }
 
 void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
