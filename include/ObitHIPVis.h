@@ -133,12 +133,14 @@ int ObitHIPGrid2HostPlanes(CUDAGridInfo *in, long ifacet, float *h_out);
 int ObitHIPGridPrepDebug(CUDAGridInfo *in, long ifacet, const float *h_vis, long long nvis,
                          int *iu, int *iv, int *itu, int *itv, float *vis);
 /* ObitUVGridFFT2Im (src/ObitUVGrid.c:673-834) on the device for every plane
- * of facet ifacet, after ObitCUDAGridFlip: one batched C2R FFT
- * (unnormalised, as FFTW's backward transform) read straight from the
- * gridder's padded half planes, then in one pass the v-half swap (as a
- * (-1)^y factor), the quadrant swap of ObitFArray2DCenter, the gridding
- * correction of GridCorrFn applied as ObitFArrayMulColRow does, and the
- * normalisation.  isBeam=1: beamNorm[p] receives each plane's centre value
+ * of facet ifacet, after ObitCUDAGridFlip: the C2R FFT (unnormalised, as
+ * FFTW's backward transform) of the gridder's padded half planes, the
+ * v-half swap (as a (-1)^y factor), the quadrant swap of
+ * ObitFArray2DCenter, the gridding correction of GridCorrFn applied as
+ * ObitFArrayMulColRow does, and the normalisation.  Sides of 1024 and 2048
+ * take two passes (a transform of every column in place, then of every row
+ * with the centring, correction and normalisation written to d_image);
+ * other sides a batched 2-D hipFFT C2R and one correcting pass.  isBeam=1: beamNorm[p] receives each plane's centre value
  * x corrections and normalises it; isBeam=0: beamNorm[p] (from the beam)
  * normalises the image.  d_image: device [nplane][ny][nx] floats.  Facets
  * must be square with a side a multiple of 8 (the reference's correction

@@ -672,6 +672,284 @@ __global__ void k_center_correct4(const float *fft, int nx, int ny, const float 
       make_float4(v.x * r * c.x, v.y * r * c.y, v.z * r * c.z, v.w * r * c.w);
 }
 
+// ---- the v transform of FFT2Image: columns of the padded half planes ----
+
+// V[k] = sum_m v[m] exp(+2 pi i m k / R) for R = 4, 8, 16, in natural
+// order: radix-2 stages on a bit-reversed copy, the stage twiddles exact
+// FP32 constants (W16^k).
+template <int R>
+__device__ __forceinline__ void dft_small(float2 (&v)[R]) {
+  constexpr float kc[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508979f,
+                            0.0f, -0.38268343236508979f, -0.70710678118654752f, -0.92387953251128674f,
+                            -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508979f,
+                            0.0f, 0.38268343236508979f, 0.70710678118654752f, 0.92387953251128674f};
+  constexpr int kBits = R == 4 ? 2 : R == 8 ? 3 : 4;
+  float2 a[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < kBits; ++b) r |= ((m >> b) & 1) << (kBits - 1 - b);
+    a[r] = v[m];
+  }
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+    const int half = len / 2;
+#pragma unroll
+    for (int i = 0; i < R; i += len) {
+#pragma unroll
+      for (int j = 0; j < half; ++j) {
+        const int e = j * (16 / len);          // W_len^j = W16^e
+        const float2 x = a[i + j], y = a[i + j + half];
+        float2 t;
+        if (e == 0) {
+          t = y;
+        } else if (e == 4) {                   // +i
+          t = make_float2(-y.y, y.x);
+        } else {
+          const float wc = kc[e], ws = kc[(e + 12) & 15];   // cos, sin(2 pi e / 16)
+          t = make_float2(y.x * wc - y.y * ws, y.x * ws + y.y * wc);
+        }
+        a[i + j] = make_float2(x.x + t.x, x.y + t.y);
+        a[i + j + half] = make_float2(x.x - t.x, x.y - t.y);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < R; ++m) v[m] = a[m];
+}
+
+// LDS layout of the transforms: point n of a column at lidx(n) = n + n / 8
+// (a free slot after every 8 points, so the stride-8 writes of a Stockham
+// pass fall on distinct banks), columns kColStride<N> float2 apart (one
+// more, so a row's CB columns fall on distinct banks too).
+__device__ __forceinline__ int lidx(int n) { return n + (n >> 3); }
+template <int N>
+constexpr int kColStride = N + N / 8 + 1;
+
+// One Stockham pass over CB columns of N points held in LDS (column c at
+// lds[c * kColStride<N>], point n at lidx(n)): for every j < N / R, with Ns
+// the length of the transforms already formed and k = j mod Ns, the R
+// points j + r N / R times W_{Ns R}^{r k}, a radix-R DFT, written to
+// (j / Ns) Ns R + k + r Ns.
+template <int N, int R, int CB>
+__device__ __forceinline__ void stockham_pass(float2 *lds, int Ns, const float2 *__restrict__ tw) {
+  constexpr int kItems = CB * N / R;
+  constexpr int kPer = (kItems + 255) / 256;
+  float2 v[kPer][R];
+  int dcol[kPer], dn[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int it = q * 256 + (int)threadIdx.x;
+    dcol[q] = -1;
+    if (it < kItems) {
+      const int c = it / (N / R), j = it - c * (N / R);
+      const int k = j & (Ns - 1);
+      const int step = N / (Ns * R);            // W_{Ns R}^{r k} = W_N^{r k step}
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = lds[c * kColStride<N> + lidx(j + r * (N / R))];
+        if (r) {
+          const float2 w = tw[(r * k * step) & (N - 1)];
+          x = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+        }
+        v[q][r] = x;
+      }
+      dft_small<R>(v[q]);
+      dcol[q] = c;
+      dn[q] = (j - k) * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kPer; ++q)
+    if (dcol[q] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[dcol[q] * kColStride<N> + lidx(dn[q] + r * Ns)] = v[q][r];
+    }
+  __syncthreads();
+}
+
+// In-place backward (exp +i) complex transform of length N along v of
+// every column of a facet's padded half planes (all pitchC = lrow / 2
+// columns of a row, the hw padding columns left of u = 0 included, so
+// blocks of 4 columns are 16-byte aligned): block (column block, plane).
+// Thread t reads rows t + (N / R1) r, r < R1, of the block's 4 columns
+// (two 16-byte loads a row) and runs the first radix-R1 pass on them in
+// registers; the two other Stockham passes (N = R1 R2 R3) go through LDS;
+// the block is written back row by row.  tw[m] = exp(+2 pi i m / N),
+// formed in FP64 on the host.
+// Blocks are dispatched to the 8 XCDs in turn, each with its own L2; block
+// x works on column block (x mod 8) * perXcd + x / 8, so an XCD takes
+// neighbouring column blocks one after another and its L2 serves the rest
+// of each 128-byte row segment that a neighbour has fetched.
+template <int N, int R1, int R2, int R3>
+__global__ __launch_bounds__(256) void k_fft_cols(float2 *grid, int pitchC, long long planeC, int nblk,
+                                                  const float2 *__restrict__ tw) {
+  constexpr int CB = 4;
+  static_assert(R1 * R2 * R3 == N && N / R1 == 256, "radices: the first pass has one item a thread");
+  __shared__ float2 lds[CB * kColStride<N>];
+  const int t = threadIdx.x;
+  const int perXcd = (nblk + 7) / 8;
+  const int blk = (int)(blockIdx.x & 7) * perXcd + (int)(blockIdx.x >> 3);
+  if (blk >= nblk) return;
+  float2 *g = grid + (long long)blockIdx.y * planeC + (long long)blk * CB;
+  float2 v[CB][R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    const float4 *src = reinterpret_cast<const float4 *>(g + (long long)(t + 256 * r) * pitchC);
+    const float4 a = src[0], b = src[1];
+    v[0][r] = make_float2(a.x, a.y);
+    v[1][r] = make_float2(a.z, a.w);
+    v[2][r] = make_float2(b.x, b.y);
+    v[3][r] = make_float2(b.z, b.w);
+  }
+  // the first pass (transforms of length 1 so far: no twiddles)
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    dft_small<R1>(v[c]);
+#pragma unroll
+    for (int r = 0; r < R1; ++r) lds[c * kColStride<N> + lidx(t * R1 + r)] = v[c][r];
+  }
+  __syncthreads();
+  stockham_pass<N, R2, CB>(lds, R1, tw);
+  stockham_pass<N, R3, CB>(lds, R1 * R2, tw);
+#pragma unroll
+  for (int i = 0; i < N / 256; ++i) {
+    const int row = t + 256 * i;
+    float4 *dst = reinterpret_cast<float4 *>(g + (long long)row * pitchC);
+    const int l = lidx(row);
+    const float2 x0 = lds[l], x1 = lds[kColStride<N> + l], x2 = lds[2 * kColStride<N> + l],
+                 x3 = lds[3 * kColStride<N> + l];
+    dst[0] = make_float4(x0.x, x0.y, x1.x, x1.y);
+    dst[1] = make_float4(x2.x, x2.y, x3.x, x3.y);
+  }
+}
+
+// The u transform of FFT2Image fused with its centring and correction:
+// a C2R of length N (= nx) of every row of the column-transformed half
+// planes, as a complex transform of length M = N / 2 (the even and odd
+// output samples as its real and imaginary parts):
+//   Z[k] = Fe[k] + i Fo[k],  Fe = X[k] + conj X[M-k],
+//   Fo = (X[k] - conj X[M-k]) exp(+2 pi i k / N),  k < M,
+//   z = backward transform of Z,  x[2m] = Re z[m],  x[2m+1] = Im z[m],
+// with the imaginary parts of X[0] and X[M] taken as zero (FFTW's C2R).
+// Output row y of plane p is input row r = (y + ny/2) mod ny times
+// (-1)^r xc[y] invNorm[p], and column x is sample (x + N/2) mod N times
+// yc[x] (ObitFArray2DCenter and ObitFArrayMulColRow, as k_center_correct4
+// applies them).  Block (4 rows, plane); 256 threads.  twN[k] = exp(+2 pi
+// i k / N) and twM = the M-point table (every other entry of twN).
+template <int N, int R1, int R2, int R3>
+__global__ __launch_bounds__(256) void k_fft_rows_c2r(const float2 *grid, int pitchC, long long planeC, int hw,
+                                                      int ny, const float2 *__restrict__ twN,
+                                                      const float2 *__restrict__ twM, const float *xc,
+                                                      const float *yc, const float *invNorm, float *img) {
+  constexpr int M = N / 2, RW = 4;
+  static_assert(R1 * R2 * R3 == M, "radices");
+  __shared__ float2 lds[RW * kColStride<M>];
+  const int t = threadIdx.x, p = blockIdx.y;
+  const int r0 = blockIdx.x * RW;
+  const float2 *g = grid + (long long)p * planeC + hw;
+  // rows r0 .. r0+3, samples 0 .. M (row c at lds[c * kColStride<M>]):
+  // every load issued before the first is waited on
+  static_assert(M % 256 == 0, "whole sample blocks per thread");
+  constexpr int kPerRow = M / 256;
+  float2 xin[RW][kPerRow];
+  float2 xlast = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int c = 0; c < RW; ++c)
+#pragma unroll
+    for (int i = 0; i < kPerRow; ++i) xin[c][i] = g[(long long)(r0 + c) * pitchC + t + 256 * i];
+  if (t < RW) xlast = g[(long long)(r0 + t) * pitchC + M];
+#pragma unroll
+  for (int c = 0; c < RW; ++c)
+#pragma unroll
+    for (int i = 0; i < kPerRow; ++i) lds[c * kColStride<M> + lidx(t + 256 * i)] = xin[c][i];
+  if (t < RW) lds[t * kColStride<M> + lidx(M)] = xlast;
+  __syncthreads();
+  // Z in place: the pair (k, M - k) by one thread; all of a thread's
+  // loads first
+  constexpr int kPairs = RW * (M / 2 + 1), kPP = (kPairs + 255) / 256;
+  float2 za[kPP], zb[kPP], wa[kPP], wb[kPP];
+#pragma unroll
+  for (int i = 0; i < kPP; ++i) {
+    const int q = t + 256 * i;
+    const int c = q / (M / 2 + 1), k = q - c * (M / 2 + 1);
+    if (q < kPairs) {
+      const float2 *row = lds + c * kColStride<M>;
+      za[i] = row[lidx(k)];
+      zb[i] = row[lidx(M - k)];
+      wa[i] = twN[k];
+      wb[i] = twN[(M - k) & (N - 1)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPP; ++i) {
+    const int q = t + 256 * i;
+    const int c = q / (M / 2 + 1), k = q - c * (M / 2 + 1);
+    if (q < kPairs) {
+      float2 *row = lds + c * kColStride<M>;
+      float2 a = za[i], b = zb[i];
+      if (k == 0) { a.y = 0.0f; b.y = 0.0f; }
+      // Z[k] from X[k], X[M-k] and exp(+2 pi i k / N)
+      auto zk = [](float2 xk, float2 xm, float2 w) {
+        const float2 fe = make_float2(xk.x + xm.x, xk.y - xm.y);
+        const float2 d = make_float2(xk.x - xm.x, xk.y + xm.y);
+        const float2 fo = make_float2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+        return make_float2(fe.x - fo.y, fe.y + fo.x);
+      };
+      row[lidx(k)] = zk(a, b, wa[i]);
+      if (k != 0) row[lidx(M - k)] = zk(b, a, wb[i]);
+    }
+  }
+  __syncthreads();
+  stockham_pass<M, R1, RW>(lds, 1, twM);
+  stockham_pass<M, R2, RW>(lds, R1, twM);
+  stockham_pass<M, R3, RW>(lds, R1 * R2, twM);
+  // centre, correct, store: thread takes output column pairs
+  const float inv = invNorm[p];
+#pragma unroll
+  for (int i = 0; i < RW * M / 256; ++i) {
+    const int q = t + 256 * i;
+    const int c = q / M, m = q - c * M;
+    const int r = r0 + c;
+    int y = r + ny / 2;
+    if (y >= ny) y -= ny;
+    const float rs = ((r & 1) ? -xc[y] : xc[y]) * inv;
+    const float2 z = lds[c * kColStride<M> + lidx(m)];
+    int x = 2 * m + N / 2;
+    if (x >= N) x -= N;
+    const float2 yy = *reinterpret_cast<const float2 *>(yc + x);
+    *reinterpret_cast<float2 *>(img + ((long long)p * ny + y) * N + x) = make_float2(z.x * rs * yy.x, z.y * rs * yy.y);
+  }
+}
+
+// Beam normalisation from the column-transformed planes: the uncorrected
+// image's (0, 0) sample is the sum of row 0's Hermitian extension,
+// X[0] + X[M] + 2 sum_{0<k<M} Re X[k] (imaginary parts of X[0], X[M]
+// dropped as the C2R drops them), times xc[ny/2] yc[nx/2]; one block a plane.
+__global__ __launch_bounds__(256) void k_beam_norm_rows(const float2 *grid, long long planeC, int hw, int nx,
+                                                        const float *xc, const float *yc, int ny, float *norm,
+                                                        float *invNorm) {
+  __shared__ double s[256];
+  const int p = blockIdx.x, M = nx / 2;
+  const float2 *row = grid + (long long)p * planeC + hw;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k <= M; k += 256) acc += (k == 0 || k == M) ? (double)row[k].x : 2.0 * (double)row[k].x;
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float v = (float)s[0] * xc[ny / 2] * yc[nx / 2];
+    norm[p] = v;
+    invNorm[p] = (float)(1.0 / fmax(1.0e-20, (double)v));
+  }
+}
+
 // Beam normalisation of every plane on the device: the centre pixel of the
 // centred image, (-1)^0 out[p][0][0] x xCorr[ny/2] x yCorr[nx/2]; norm and
 // its reciprocal (as the reference forms it, 1 / max(1e-20, norm)).
@@ -741,6 +1019,8 @@ struct GpuState {
   float *h_norm = nullptr;  // pinned: plane norms / reciprocals staged for the caller
   float *d_fft = nullptr;
   size_t fftCap = 0;
+  float2 *d_tw = nullptr;          // exp(+2 pi i m / twN) for k_fft_cols
+  int twN = 0;
   float *d_corr = nullptr;  // xCorr[nx], yCorr[ny], 1/norm[nplane]
   size_t corrCap = 0;
   std::vector<float> h_corr;  // xCorr, yCorr of the last (nx, ny) (GridCorrFn)
@@ -1317,7 +1597,7 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
     (void)hipFree(gs->d_itemOff); (void)hipFree(gs->d_scanTmp);
     if (gs->h_tot) (void)hipHostFree(gs->h_tot);
     if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
-    (void)hipFree(gs->d_fft); (void)hipFree(gs->d_corr);
+    (void)hipFree(gs->d_fft); (void)hipFree(gs->d_corr); (void)hipFree(gs->d_tw);
     if (gs->h_norm) (void)hipHostFree(gs->h_norm);
     if (gi && gi->stream && gi->stream[iGPU] == (int *)gs->stream) gi->stream[iGPU] = nullptr;
     (void)hipStreamDestroy(gs->stream);
@@ -1428,40 +1708,6 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
                       "the reference's correction indexes rows and columns alike", nx, ny);
   if (!gi->h_convfn) return obit::fail(obit::kState, "no convolving function");
   if (nplane > 1024) return obit::fail(obit::kInvalid, "%d planes: at most 1024", nplane);
-  // real output of the C2R for every plane
-  const size_t need = (size_t)nplane * ny * nx;
-  if (need > gs->fftCap) {
-    (void)hipFree(gs->d_fft);
-    gs->d_fft = nullptr;
-    gs->fftCap = 0;
-    OBIT_HIP_TRY(hipMalloc(&gs->d_fft, sizeof(float) * need));
-    gs->fftCap = need;
-  }
-  // one batched C2R (FFTW's unnormalised backward transform) read straight
-  // from the gridder's padded half planes: row pitch lrow/2 complex, the
-  // u = 0 column hw complex into each row, planes sizeGrid floats apart
-  const int pitchC = f.lrow / 2;
-  if (!gs->fftPlan || gs->fftNx != nx || gs->fftNy != ny || gs->fftNplane != nplane ||
-      gs->fftPitch != pitchC || gs->fftDist != f.sizeGrid) {
-    if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
-    gs->fftPlan = 0;
-    int n[2] = {ny, nx}, inembed[2] = {ny, pitchC}, onembed[2] = {ny, nx};
-    hipfftResult r = hipfftPlanMany(&gs->fftPlan, 2, n, inembed, 1, f.sizeGrid / 2, onembed, 1, ny * nx,
-                                    HIPFFT_C2R, nplane);
-    if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftPlanMany: %d", (int)r);
-    gs->fftNx = nx;
-    gs->fftNy = ny;
-    gs->fftNplane = nplane;
-    gs->fftPitch = pitchC;
-    gs->fftDist = f.sizeGrid;
-  }
-  if (hipfftSetStream(gs->fftPlan, gs->stream) != HIPFFT_SUCCESS)
-    return obit::fail(obit::kHipError, "hipfftSetStream");
-  {
-    hipfftComplex *src = reinterpret_cast<hipfftComplex *>(f.grid + 2 * f.hw);
-    hipfftResult r = hipfftExecC2R(gs->fftPlan, src, reinterpret_cast<hipfftReal *>(gs->d_fft));
-    if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftExecC2R: %d", (int)r);
-  }
   // gridding corrections (GridCorrFn, computed once per size): xCorr[nx],
   // yCorr[ny], then the planes' norms and reciprocals
   const size_t ncorr = (size_t)nx + ny + 2 * 1024;
@@ -1484,16 +1730,102 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
   if (!gs->h_norm) OBIT_HIP_TRY(hipHostMalloc(&gs->h_norm, sizeof(float) * 2 * 1024, hipHostMallocDefault));
   float *d_norm = gs->d_corr + nx + ny, *d_inv = d_norm + 1024;
   const float *xc = gs->d_corr, *yc = gs->d_corr + nx;
-  if (isBeam) {
-    hipLaunchKernelGGL(obit::grid::k_beam_norm, dim3((nplane + 63) / 64), dim3(64), 0, gs->stream, gs->d_fft, nx,
-                       ny, nplane, xc, yc, d_norm, d_inv);
-    OBIT_HIP_TRY(hipGetLastError());
-  } else {
+  if (!isBeam) {
+    // an image takes its beam's norms
     OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));  // the pinned staging is free
     for (int p = 0; p < nplane; ++p) gs->h_norm[p] = (float)(1.0 / std::max(1.0e-20, (double)beamNorm[p]));
     OBIT_HIP_TRY(hipMemcpyAsync(d_inv, gs->h_norm, sizeof(float) * nplane, hipMemcpyHostToDevice, gs->stream));
   }
-  {
+  // FFTW's unnormalised backward transform of the gridder's padded half
+  // planes (row pitch lrow/2 complex, the u = 0 column hw complex into each
+  // row, planes sizeGrid floats apart), then ObitFArray2DCenter's quadrant
+  // swap and ObitFArrayMulColRow's correction.  For a side of 1024 or 2048:
+  // k_fft_cols transforms every column along v in place (the grid is
+  // consumed), a beam's norm is summed from row 0 of the result, and
+  // k_fft_rows_c2r transforms every row along u, centres, corrects and
+  // writes the image: two passes over the planes.  Other sizes: one batched
+  // 2-D C2R (hipFFT) into scratch, then k_center_correct4.
+  const int pitchC = f.lrow / 2;
+  const bool own = (nx == 1024 || nx == 2048) && pitchC % 4 == 0 && f.sizeGrid % 8 == 0 && ny % 4 == 0;
+  if (own) {
+    // exp(+2 pi i m / n) for m < n, and for m < n / 2 at n / 2, in FP64
+    if (gs->twN != ny) {
+      std::vector<float2> tw((size_t)ny + ny / 2);
+      for (int m = 0; m < ny; ++m) {
+        const double a = 2.0 * M_PI * (double)m / (double)ny;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+      for (int m = 0; m < ny / 2; ++m) tw[ny + m] = tw[2 * m];
+      (void)hipFree(gs->d_tw);
+      gs->d_tw = nullptr;
+      gs->twN = 0;
+      OBIT_HIP_TRY(hipMalloc(&gs->d_tw, sizeof(float2) * tw.size()));
+      OBIT_HIP_TRY(hipMemcpy(gs->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+      gs->twN = ny;
+    }
+    const float2 *twN = gs->d_tw, *twM = gs->d_tw + ny;
+    float2 *g2 = reinterpret_cast<float2 *>(f.grid);
+    const long long planeC = (long long)f.sizeGrid / 2;
+    // every column of the padded rows, 4 at a time (8 x ceil(blocks / 8)
+    // launched; the spare ones return at once)
+    const int nblk = pitchC / 4;
+    const dim3 gcol((unsigned)(8 * ((nblk + 7) / 8)), (unsigned)nplane);
+    const dim3 grow((unsigned)(ny / 4), (unsigned)nplane);
+    if (nx == 2048)
+      hipLaunchKernelGGL((obit::grid::k_fft_cols<2048, 8, 16, 16>), gcol, dim3(256), 0, gs->stream, g2, pitchC,
+                         planeC, nblk, twN);
+    else
+      hipLaunchKernelGGL((obit::grid::k_fft_cols<1024, 4, 16, 16>), gcol, dim3(256), 0, gs->stream, g2, pitchC,
+                         planeC, nblk, twN);
+    OBIT_HIP_TRY(hipGetLastError());
+    if (isBeam) {
+      hipLaunchKernelGGL(obit::grid::k_beam_norm_rows, dim3((unsigned)nplane), dim3(256), 0, gs->stream,
+                         (const float2 *)g2, planeC, f.hw, nx, xc, yc, ny, d_norm, d_inv);
+      OBIT_HIP_TRY(hipGetLastError());
+    }
+    if (nx == 2048)
+      hipLaunchKernelGGL((obit::grid::k_fft_rows_c2r<2048, 4, 16, 16>), grow, dim3(256), 0, gs->stream,
+                         (const float2 *)g2, pitchC, planeC, f.hw, ny, twN, twM, xc, yc, (const float *)d_inv,
+                         d_image);
+    else
+      hipLaunchKernelGGL((obit::grid::k_fft_rows_c2r<1024, 8, 8, 8>), grow, dim3(256), 0, gs->stream,
+                         (const float2 *)g2, pitchC, planeC, f.hw, ny, twN, twM, xc, yc, (const float *)d_inv,
+                         d_image);
+    OBIT_HIP_TRY(hipGetLastError());
+  } else {
+    // real output of the C2R for every plane
+    const size_t need = (size_t)nplane * ny * nx;
+    if (need > gs->fftCap) {
+      (void)hipFree(gs->d_fft);
+      gs->d_fft = nullptr;
+      gs->fftCap = 0;
+      OBIT_HIP_TRY(hipMalloc(&gs->d_fft, sizeof(float) * need));
+      gs->fftCap = need;
+    }
+    if (!gs->fftPlan || gs->fftNx != nx || gs->fftNy != ny || gs->fftNplane != nplane ||
+        gs->fftPitch != pitchC || gs->fftDist != f.sizeGrid) {
+      if (gs->fftPlan) (void)hipfftDestroy(gs->fftPlan);
+      gs->fftPlan = 0;
+      int n[2] = {ny, nx}, inembed[2] = {ny, pitchC}, onembed[2] = {ny, nx};
+      hipfftResult r = hipfftPlanMany(&gs->fftPlan, 2, n, inembed, 1, f.sizeGrid / 2, onembed, 1, ny * nx,
+                                      HIPFFT_C2R, nplane);
+      if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftPlanMany: %d", (int)r);
+      gs->fftNx = nx;
+      gs->fftNy = ny;
+      gs->fftNplane = nplane;
+      gs->fftPitch = pitchC;
+      gs->fftDist = f.sizeGrid;
+    }
+    if (hipfftSetStream(gs->fftPlan, gs->stream) != HIPFFT_SUCCESS)
+      return obit::fail(obit::kHipError, "hipfftSetStream");
+    hipfftComplex *src = reinterpret_cast<hipfftComplex *>(f.grid + 2 * f.hw);
+    hipfftResult r = hipfftExecC2R(gs->fftPlan, src, reinterpret_cast<hipfftReal *>(gs->d_fft));
+    if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftExecC2R: %d", (int)r);
+    if (isBeam) {
+      hipLaunchKernelGGL(obit::grid::k_beam_norm, dim3((nplane + 63) / 64), dim3(64), 0, gs->stream, gs->d_fft, nx,
+                         ny, nplane, xc, yc, d_norm, d_inv);
+      OBIT_HIP_TRY(hipGetLastError());
+    }
     dim3 grid((unsigned)((nx / 4 + 255) / 256), (unsigned)ny, (unsigned)nplane);
     hipLaunchKernelGGL(obit::grid::k_center_correct4, grid, dim3(256), 0, gs->stream, gs->d_fft, nx, ny, xc, yc,
                        d_inv, d_image);

@@ -151,15 +151,17 @@ def test_grid_device_path_matches_buffers(uv4):
         assert np.array_equal(y, z)
 
 
-def test_fft2image_matches_oracle(oracle, uv4):
-    """Beam and image of a 1024^2 facet through the device FFT, quadrant
+@pytest.mark.parametrize("nx", [1024, 2048, 512])
+def test_fft2image_matches_oracle(oracle, uv4, nx):
+    """Beam and image of an nx^2 facet through the device FFT, quadrant
     swap, correction and normalisation (ObitUVGridFFT2Im) against the
     oracle's FP64 restatement applied to the device's own planes, with the
     library's correction functions (pinned separately against their
-    definition)."""
+    definition).  1024 and 2048 take the column kernel and a batched 1-D
+    C2R, 512 the batched 2-D C2R."""
     import ctypes as C
     from obit_amd import _lib
-    f = _facets_cfg4(1)
+    f = _facets_cfg4(1, nx)
     cf = oracle.convfunc()[0]
     n = f[0].nx
     corr = np.zeros(n, np.float32)
