@@ -288,9 +288,9 @@ def bench_fft2image(lib, gg, facets, regrid):
     finally:
         lib.ObitHIPFree(d)
     planes = len(facets) * nplane
-    # per plane at least: the half-plane grid read and the real output
-    # written by the C2R, then that output read and the image written
-    mib = planes * f.ny * (f.nx // 2 + 1) * 8 / 2**20 + planes * f.ny * f.nx * 4 * 3 / 2**20
+    # per plane at least: the half-plane grid read and written by the
+    # column transform, read by the row transform, and the image written
+    mib = planes * f.ny * (f.nx // 2 + 1) * 8 * 3 / 2**20 + planes * f.ny * f.nx * 4 / 2**20
     log(f"fft to image: {planes} planes of {f.nx}x{f.ny} in {ms:.1f} ms")
     return {"planes": planes, "ms": ms, "planes_per_s": planes / (ms * 1e-3),
             "min_traffic_GBps": mib * 2**20 / (ms * 1e-3) / 1e9}
