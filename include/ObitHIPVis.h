@@ -59,6 +59,12 @@ int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq
  * freqScale is read again whenever its pointer changes or a table has been
  * uploaded through ObitCUDAUtilHost2GPU / ObitCUDAUtilHostAny2GPU since. */
 int ObitHIPSkyModelDFTGetPath(GPUInfo *gpuInfo, int *path);
+/* Shape of the last ObitCUDASkyModelDFTCalc launch: channels per thread
+ * and the number of component parts whose sums a second kernel adds (1:
+ * none).  A buffer of few rows makes few (256-row, IF, channel chunk)
+ * work items, so DFTCalc also cuts the component list into parts to give
+ * every CU an equal share. */
+int ObitHIPSkyModelDFTGetBufferShape(GPUInfo *gpuInfo, int *cpt, int *nsplit);
 /* CalcResident's operation on nvis rows at d_vis (the caller's device
  * buffer) with the component table d_model (the caller's device memory,
  * ncomp rows of h_modelInfo->size floats), without allocating anything.

@@ -216,6 +216,7 @@ _PROTOS = {
     "ObitHIPSkyModelDFTSetFreq": (_I, [_P, _DP, _I, C.c_double]),
     "ObitHIPSkyModelDFTSetSpecRef": (_I, [_P, _DP, _I]),
     "ObitHIPSkyModelDFTGetPath": (_I, [_P, C.POINTER(C.c_int)]),
+    "ObitHIPSkyModelDFTGetBufferShape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ObitHIPSkyModelDFTCalcNoAlloc": (_I, [_P, _P, _I, _P, _LL]),
     "ObitHIPSkyModelDFTSetVis": (_I, [_P, _P, _LL]),
     "ObitHIPSkyModelDFTAttachVis": (_I, [_P, _P, _LL]),

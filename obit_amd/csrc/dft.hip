@@ -68,6 +68,11 @@ struct Desc {
   int tab_ldk, tab_nchpad;
   int op, doSwap;
   float stokFactor;
+  // component parts: with nsplit > 1 an item takes one part of kpart
+  // components and writes its FP64 sums to part ([nsplit][nrow][nIFsel]
+  // [nchsel] complex) for k_dft_combine, instead of applying them
+  int nsplit, kpart;
+  double2 *part;
 };
 
 template <int PREC>
@@ -119,7 +124,12 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
 
   for (long long item = blockIdx.x; item < d.nitems; item += gridDim.x) {
     const int rb = (int)(item % d.nrowblk);
-    const int ic = (int)(item / d.nrowblk);
+    const long long q = item / d.nrowblk;
+    const int nic = d.nIFsel * d.nchunk;
+    const int ic = (int)(q % nic);
+    const int part = (int)(q / nic);
+    const int kBeg = d.nsplit > 1 ? part * d.kpart : 0;
+    const int kEnd = d.nsplit > 1 ? min(d.ncomp, kBeg + d.kpart) : d.ncomp;
     const int isel = ic / d.nchunk;
     const int c0 = (ic - isel * d.nchunk) * CPT;
     const long long row = (long long)rb * kRows + tid;
@@ -139,11 +149,11 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) { accr[j] = acci[j] = 0.0f; outr[j] = outi[j] = 0.0; }
 
-    for (int k0 = 0; k0 < d.ncomp; k0 += TILE) {
+    for (int k0 = kBeg; k0 < kEnd; k0 += TILE) {
       __syncthreads();
       for (int t = tid; t < TILE; t += kRows) {
         const int k = k0 + t;
-        const bool ok = k < d.ncomp;
+        const bool ok = k < kEnd;
         s_x[t] = ok ? d.cx[k] : 0.0;
         s_y[t] = ok ? d.cy[k] : 0.0;
         s_z[t] = ok ? d.cz[k] : 0.0;
@@ -158,13 +168,13 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
         for (int t = tid; t < TILE * CPT; t += kRows) {
           const int kk = t / CPT, cc = t - kk * CPT;
           const int k = k0 + kk;
-          s_t[t] = (k < d.ncomp)
+          s_t[t] = (k < kEnd)
                        ? d.tab[(size_t)k * d.tab_ldk + (size_t)isel * d.tab_nchpad + c0 + cc]
                        : 0.0f;
         }
       }
       __syncthreads();
-      const int kn = min(TILE, d.ncomp - k0);
+      const int kn = min(TILE, kEnd - k0);
       for (int kk = 0; kk < kn; ++kk) {
         const double psi = fma(w, s_z[kk], fma(v, s_y[kk], u * s_x[kk]));
         double pm = psi * fm, pd = psi * fd;
@@ -240,7 +250,13 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
       }
     }
 
-    if (valid) {
+    if (valid && d.part) {
+      double2 *pp = d.part + (((long long)part * d.nrow + row) * d.nIFsel + isel) * d.nchsel + c0;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        if (c0 + j < d.nchsel) pp[j] = make_double2(outr[j], outi[j]);
+      }
+    } else if (valid) {
       float *p = d.vis + row * (long long)d.lrec + d.nrparm + (d.IFb + isel) * d.incif +
                  (d.chanb + c0) * d.incf + d.stokb * d.incs;
 #pragma unroll
@@ -248,6 +264,30 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
         if (c0 + j < d.nchsel) apply_op(d, p + j * d.incf, (float)outr[j], (float)outi[j]);
       }
     }
+  }
+}
+
+// The sums of the nsplit component parts of every (row, IF, channel),
+// added in part order in FP64, applied to the row as k_dft_split applies
+// its own.
+__global__ __launch_bounds__(256) void k_dft_combine(Desc d) {
+  const long long n = d.nrow * d.nIFsel * d.nchsel;
+  const long long slab = n;  // entries of one part
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ch = (int)(i % d.nchsel);
+    const long long q = i / d.nchsel;
+    const int isel = (int)(q % d.nIFsel);
+    const long long row = q / d.nIFsel;
+    double mr = 0.0, mi = 0.0;
+    for (int s = 0; s < d.nsplit; ++s) {
+      const double2 v = d.part[(long long)s * slab + i];
+      mr += v.x;
+      mi += v.y;
+    }
+    float *p = d.vis + row * (long long)d.lrec + d.nrparm + (d.IFb + isel) * d.incif +
+               (d.chanb + ch) * d.incf + d.stokb * d.incs;
+    apply_op(d, p, (float)mr, (float)mi);
   }
 }
 
@@ -426,6 +466,7 @@ struct State {
   unsigned long long fscaleGen = ~0ULL;    // ... after this many table uploads
   bool fallbackNoted = false;     // the FP64 fallback message was printed
   int lastPath = 0;               // kernel of the last launch: 0 split, 1 FP64
+  int lastCpt = 0, lastSplit = 0;  // shape of the last per-buffer launch
   double *dfreq = nullptr;        // [2*nIFsel]: fbase, fdel
   double *hfreq = nullptr;        // pinned staging
   int freqCap = 0;
@@ -436,6 +477,10 @@ struct State {
   bool ownVis = false;
   float *stage = nullptr;
   size_t stageCap = 0;
+  double2 *part = nullptr;  // per-part sums of the per-buffer path
+  size_t partCap = 0;
+  std::vector<hipEvent_t> events;                 // per-buffer segment events
+  std::unordered_map<const void *, int> occ;      // resident workgroups per CU of a kernel
   // kernel shape
   int cpt = 32, tile = 64, wgPerCU = 2, prec = 1;
   // graph
@@ -700,12 +745,16 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
   return kOK;
 }
 
+// wgPerCU > 0: a persistent grid of wgPerCU workgroups per CU striding over
+// the items; wgPerCU = 0: one workgroup per item, placed by the dispatcher
+// as CUs free up.
 int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipStream_t stream = nullptr) {
   if (d.nitems <= 0) return kOK;
   const bool g = model_gauss_off(st->type) >= 0, s = model_is_spec(st->type);
   KernelFn fn = pick_split(cpt, tile, g, s, st->prec);
-  long long grid = (long long)st->nCU * wgPerCU;
+  long long grid = wgPerCU > 0 ? (long long)st->nCU * wgPerCU : d.nitems;
   if (grid > d.nitems) grid = d.nitems;
+  if (grid > INT_MAX) return fail(kInvalid, "%lld work items exceed one launch", d.nitems);
   Desc dd = d;
   void *args[] = {&dd};
   OBIT_HIP_TRY(hipLaunchKernel((const void *)fn, dim3((unsigned)grid), dim3(kRows), args, 0,
@@ -742,6 +791,83 @@ int run_device(State *st, const GPUInfo *gi, float *vis, long long nrow, bool fp
   st->lastPath = (fp64 || !st->uniform) ? 1 : 0;
   if (fp64 || !st->uniform) return launch_fp64(st, d);
   return launch_split(st, d, st->cpt, st->tile, st->wgPerCU);
+}
+
+// ---- the per-buffer launch plan (ObitCUDASkyModelDFTCalc) ----
+
+// Resident workgroups per CU of a kernel (cached).
+int occupancy(State *st, KernelFn fn) {
+  const void *key = (const void *)fn;
+  auto it = st->occ.find(key);
+  if (it != st->occ.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, key, kRows, 0) != hipSuccess || nb < 1) nb = 1;
+  st->occ[key] = nb;
+  return nb;
+}
+
+struct BufPlan {
+  int cpt = 32, tile = 64, nsplit = 1, kpart = 1;
+  bool serial = false;      // each segment's kernel waits for the one before
+  std::vector<int> segBlk;  // row blocks of each segment
+};
+
+// The plan of one per-buffer call.  The kernel is bound by the VALU of
+// each CU, and a buffer of nVisPIO rows makes few (256-row block, IF,
+// channel chunk) items: 320 for a 10,000-row buffer of 4 IFs x 64 channels
+// at 32 channels per thread, against 512 resident workgroups.  So:
+//   * channels per thread: 32, or 16 where that wastes fewer padded
+//     channels (a 16-channel chunk runs at about 0.9 of the rate);
+//   * the component list is cut into nsplit parts of whole 64-component
+//     tiles (4 tiles at least), so the buffer makes about 10 rounds of
+//     resident workgroups, each placed by the dispatcher as a CU frees up;
+//     their FP64 sums go to a scratch (at most 1 GiB) that k_dft_combine
+//     adds in part order and applies;
+//   * segments of whole rounds: one round first (its upload is the only
+//     one nothing overlaps), two rounds each in the middle, and a short
+//     last one (its download is the only one nothing overlaps).
+// Measured on workload 2's 10,000-row buffers (4 IFs x 64 channels, 10,000
+// components; 2 workgroups per CU resident): 16 parts and segments of
+// 4, 8, 8, 8, 8, 4 row blocks ran at 5.7e12 evaluations/s, against 5.5e12
+// with five equal segments, 4.9e12 with three, and 5.3-5.45e12 with 12,
+// 23 or 32 parts.
+BufPlan plan_buffer(State *st, const Desc &d, size_t rowBytes) {
+  (void)rowBytes;
+  BufPlan p;
+  p.kpart = std::max(d.ncomp, 1);
+  const int n16 = (d.nchsel + 15) / 16, n32 = (d.nchsel + 31) / 32;
+  p.cpt = (n16 * 16 / 0.9 < n32 * 32.0) ? 16 : 32;
+  const int nchunk = p.cpt == 16 ? n16 : n32;
+  const bool g = model_gauss_off(st->type) >= 0, s = model_is_spec(st->type);
+  const long long slots = (long long)std::max(st->nCU, 1) * occupancy(st, pick_split(p.cpt, p.tile, g, s, st->prec));
+  const long long perBlock = (long long)d.nIFsel * nchunk;  // items of one row block, one part
+  const long long items = perBlock * d.nrowblk;
+  if (d.ncomp > 0) {
+    const double chans = (double)d.nrow * d.nIFsel * d.nchsel;
+    int want = (int)std::min<long long>(16, std::max<long long>(1, (10 * slots + items - 1) / items));
+    for (; want > 1; --want) {
+      const int kpt = ((d.ncomp + want - 1) / want + p.tile - 1) / p.tile * p.tile;
+      const int ns = (d.ncomp + kpt - 1) / kpt;
+      if (kpt < 4 * p.tile || (double)ns * chans * sizeof(double2) > double(size_t(1) << 30)) continue;
+      p.nsplit = ns;
+      p.kpart = kpt;
+      break;
+    }
+  }
+  const int round = (int)std::max<long long>(1, slots / (perBlock * p.nsplit));  // row blocks per round
+  const int mid = std::max(2 * round, (d.nrowblk + 15) / 16);                     // at most ~17 segments
+  int b = 0;
+  for (int i = 0; b < d.nrowblk; ++i) {
+    int want = (i == 0) ? round : mid;
+    const int left = d.nrowblk - b;
+    // the last segment: what is left, at most one round; the one before
+    // leaves it
+    if (left <= want) want = left;
+    else if (left < want + round) want = left - round;
+    p.segBlk.push_back(want);
+    b += want;
+  }
+  return p;
 }
 
 // ---- the call that allocates nothing (ObitHIPSkyModelDFTCalcNoAlloc) ----
@@ -928,26 +1054,9 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
     OBIT_HIP_CHECK(hipMalloc(&st->stage, bytes));
     st->stageCap = bytes;
   }
-  // channels per thread for this buffer: the kernel is bound by the VALU
-  // of each CU, so a launch lasts about as long as its busiest CU's share,
-  // ceil(items / CUs) items that each cost about cpt / e(cpt) (e = 1, 0.9,
-  // 0.6 for 32, 16, 8 channels: the per-chunk phasor set-up weighs more on
-  // short chunks); a 10,000-row buffer of 4 IFs x 64 channels has 320
-  // items at 32 channels (2 on the busiest CU: 64), 640 at 16 (3: 53), 1,280
-  // at 8 (5: 67)
   Desc d;
   int rc = make_desc(st, gpuInfo, st->stage, nvis, 32, d);
   if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
-  const long long ncu = std::max(st->nCU, 1);
-  int cpt = 32, tile = st->tile;
-  double best = 0.0;
-  for (int c : {32, 16, 8}) {
-    const long long items = (long long)d.nrowblk * d.nIFsel * ((d.nchsel + c - 1) / c);
-    const double cost = (double)((items + ncu - 1) / ncu) * c / (c == 32 ? 1.0 : c == 16 ? 0.9 : 0.6);
-    if (c == 32 || cost < best) { best = cost; cpt = c; }
-  }
-  if (cpt == 8) tile = 64;
-  d.nchunk = (d.nchsel + cpt - 1) / cpt;
   if (!st->uniform) {
     // the FP64 kernel, in one piece on the library's stream
     OBIT_HIP_CHECK(hipMemcpyAsync(st->stage, gpuInfo->h_data, bytes, hipMemcpyHostToDevice, st->stream));
@@ -957,53 +1066,79 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
     OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
     return;
   }
+  const BufPlan plan = plan_buffer(st, d, rowBytes);
+  const int cpt = plan.cpt, nsplit = plan.nsplit, nseg = (int)plan.segBlk.size();
+  d.nchunk = (d.nchsel + cpt - 1) / cpt;
+  if (nsplit > 1) {
+    const size_t need = (size_t)nsplit * (size_t)nvis * d.nIFsel * d.nchsel * sizeof(double2);
+    if (need > st->partCap) {
+      if (st->part) OBIT_HIP_CHECK(hipFree(st->part));
+      st->part = nullptr;
+      st->partCap = 0;
+      OBIT_HIP_CHECK(hipMalloc(&st->part, need));
+      st->partCap = need;
+    }
+  }
   st->lastPath = 0;
-  // the buffer in up to 3 row segments on the caller's streams
-  // (ObitGPUSkyModelDFTInit creates them): each segment's upload, launch
-  // and download queue on its own stream, so one segment's kernel runs
-  // while the next is uploaded and the previous downloaded (as the
-  // reference's ObitCUDASkyModel.cu:804-963 pipelines its streams).  The
-  // uploads are chained (each waits for the one before) so the first
-  // segment's rows arrive at the full link rate; a fourth stream would
-  // share a hardware queue with another (4 queues per process, one taken
-  // by the library's own stream) and run behind it.
-  int nseg = 0;
+  st->lastCpt = cpt;
+  st->lastSplit = nsplit;
+  // The buffer in row segments on the caller's streams (ObitGPUSkyModel-
+  // DFTInit creates them), round robin: each segment's upload, launch and
+  // download queue on its stream, so kernels run while later segments are
+  // uploaded and earlier ones downloaded (as the reference's
+  // ObitCUDASkyModel.cu:804-963 pipelines its streams).  The uploads are
+  // chained (each waits for the one before) so every segment's rows arrive
+  // at the full link rate; with plan.serial each kernel also waits for the
+  // one before, so segments finish, and download, in order.  At most 3
+  // streams: a fourth would share a hardware queue with another (4 queues
+  // per process, one taken by the library's own stream) and run behind it.
+  int nstr = 0;
   hipStream_t segs[3];
-  for (int i = 0; i < gpuInfo->nstream && i < STREAM_COUNT && nseg < 3; ++i)
-    if (gpuInfo->stream && gpuInfo->stream[i]) segs[nseg++] = reinterpret_cast<hipStream_t>(gpuInfo->stream[i]);
-  if (nseg == 0) segs[nseg++] = st->stream;
-  // whole row blocks per segment, every segment non-empty
-  nseg = (int)std::min<long long>(nseg, d.nrowblk);
-  const long long blkPer = (d.nrowblk + nseg - 1) / nseg;
-  nseg = (int)((d.nrowblk + blkPer - 1) / blkPer);
+  for (int i = 0; i < gpuInfo->nstream && i < STREAM_COUNT && nstr < 3; ++i)
+    if (gpuInfo->stream && gpuInfo->stream[i]) segs[nstr++] = reinterpret_cast<hipStream_t>(gpuInfo->stream[i]);
+  if (nstr == 0) segs[nstr++] = st->stream;
+  while ((int)st->events.size() < 2 * nseg + 1) {
+    hipEvent_t e;
+    OBIT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    st->events.push_back(e);
+  }
+  hipEvent_t ready = st->events[0];
+  hipEvent_t *up = st->events.data() + 1, *kd = st->events.data() + 1 + nseg;
   // the frequency lines (and any spectral table) were queued on the
   // library's stream: the segments start after them
-  hipEvent_t ready, up[3];
-  OBIT_HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-  for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipEventCreateWithFlags(&up[i], hipEventDisableTiming));
   OBIT_HIP_CHECK(hipEventRecord(ready, st->stream));
-  for (int i = 0; i < nseg; ++i) {
-    const long long r0 = i * blkPer * kRows;
-    const long long n = std::min(blkPer * kRows, nvis - r0);
-    hipStream_t s = segs[i];
+  long long r0 = 0;
+  for (int i = 0; i < nseg && r0 < nvis; ++i) {
+    const long long n = std::min((long long)plan.segBlk[i] * kRows, nvis - r0);
+    hipStream_t s = segs[i % nstr];
     OBIT_HIP_CHECK(hipStreamWaitEvent(s, i ? up[i - 1] : ready, 0));
     float *dv = st->stage + r0 * gpuInfo->lenvis;
     OBIT_HIP_CHECK(hipMemcpyAsync(dv, gpuInfo->h_data + r0 * gpuInfo->lenvis, (size_t)n * rowBytes,
                                   hipMemcpyHostToDevice, s));
     OBIT_HIP_CHECK(hipEventRecord(up[i], s));
+    if (plan.serial && i) OBIT_HIP_CHECK(hipStreamWaitEvent(s, kd[i - 1], 0));
     Desc ds = d;
     ds.vis = dv;
     ds.nrow = n;
     ds.nrowblk = (int)((n + kRows - 1) / kRows);
-    ds.nitems = (long long)ds.nrowblk * ds.nIFsel * ds.nchunk;
-    rc = launch_split(st, ds, cpt, tile, st->wgPerCU, s);
+    ds.nsplit = nsplit;
+    ds.kpart = plan.kpart;
+    ds.part = nsplit > 1 ? st->part + (size_t)nsplit * (size_t)r0 * d.nIFsel * d.nchsel : nullptr;
+    ds.nitems = (long long)ds.nrowblk * ds.nIFsel * ds.nchunk * nsplit;
+    rc = launch_split(st, ds, cpt, plan.tile, 0, s);
+    if (!rc && nsplit > 1) {
+      const long long m = n * ds.nIFsel * ds.nchsel;
+      const int nb = (int)std::min<long long>((m + 255) / 256, (long long)st->nCU * 8);
+      hipLaunchKernelGGL(k_dft_combine, dim3(nb), dim3(256), 0, s, ds);
+      if (hipGetLastError() != hipSuccess) rc = obit::fail(obit::kHipError, "k_dft_combine launch");
+    }
     if (rc) { std::fprintf(stderr, "obit_hip: DFTCalc: %s\n", ObitHIPLastError()); std::abort(); }
+    OBIT_HIP_CHECK(hipEventRecord(kd[i], s));
     OBIT_HIP_CHECK(hipMemcpyAsync(gpuInfo->h_data + r0 * gpuInfo->lenvis, dv, (size_t)n * rowBytes,
                                   hipMemcpyDeviceToHost, s));
+    r0 += n;
   }
-  for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipStreamSynchronize(segs[i]));
-  OBIT_HIP_CHECK(hipEventDestroy(ready));
-  for (int i = 0; i < nseg; ++i) OBIT_HIP_CHECK(hipEventDestroy(up[i]));
+  for (int i = 0; i < nstr; ++i) OBIT_HIP_CHECK(hipStreamSynchronize(segs[i]));
 }
 
 void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
@@ -1018,7 +1153,8 @@ void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
   (void)hipStreamSynchronize(st->stream);
   obit::dft::destroy_graph(st);
   (void)hipFree(st->cx); (void)hipFree(st->cy); (void)hipFree(st->cz); (void)hipFree(st->camp); (void)hipFree(st->cg);
-  (void)hipFree(st->tab); (void)hipFree(st->dfreq); (void)hipFree(st->stage);
+  (void)hipFree(st->tab); (void)hipFree(st->dfreq); (void)hipFree(st->stage); (void)hipFree(st->part);
+  for (hipEvent_t e : st->events) (void)hipEventDestroy(e);
   if (st->hfreq) (void)hipHostFree(st->hfreq);
   if (st->ownVis) (void)hipFree(st->dvis);
   (void)hipStreamDestroy(st->stream);
@@ -1035,6 +1171,15 @@ int ObitHIPSkyModelDFTSetFreq(GPUInfo *gpuInfo, const double *freqArr, int nfreq
   st->uvRef = uvRefFreq;
   st->tabReady = false;
   obit::dft::destroy_graph(st);
+  return obit::kOK;
+}
+
+int ObitHIPSkyModelDFTGetBufferShape(GPUInfo *gpuInfo, int *cpt, int *nsplit) {
+  State *st = find_state(gpuInfo);
+  if (!st) return obit::fail(obit::kState, "DFTGetBufferShape before DFTInit");
+  if (!cpt || !nsplit) return obit::fail(obit::kInvalid, "NULL output");
+  *cpt = st->lastCpt;
+  *nsplit = st->lastSplit;
   return obit::kOK;
 }
 

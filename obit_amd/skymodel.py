@@ -227,6 +227,14 @@ class SkyModelDFT:
         _lib.check(self.lib.ObitHIPSkyModelDFTGetPath(self.g, C.byref(p)), "GetPath")
         return p.value
 
+    def buffer_shape(self) -> tuple[int, int]:
+        """(channels per thread, component parts) of the last per-buffer
+        launch (calc)."""
+        a, b = C.c_int(), C.c_int()
+        _lib.check(self.lib.ObitHIPSkyModelDFTGetBufferShape(self.g, C.byref(a), C.byref(b)),
+                   "GetBufferShape")
+        return a.value, b.value
+
     def set_precision(self, prec: int) -> None:
         _lib.check(self.lib.ObitHIPSkyModelDFTSetPrecision(self.g, prec), "SetPrecision")
 

@@ -344,3 +344,60 @@ def test_nonuniform_channels_report_fp64_kernel(oracle, freq64):
     err = np.abs(_model_from(uv, out) - ref).max() / np.abs(ref).max()
     print(f"non-uniform table (freq64={freq64}): max {err:.3e}", flush=True)
     assert err <= 2e-7  # FP32 output rounding only
+
+
+def _split_sky(kind, uv, sky0, rng):
+    """A model of the given kind on workload 2's layout (4 IF x 64 ch)."""
+    k = sky0.comps.shape[0]
+    base = sky0.comps[:, :4]
+    if kind == "point":
+        return sky0
+    if kind == "gauss":
+        comps = np.concatenate([base, _gauss_cols(k, rng)], axis=1).astype(np.float32)
+        return synth.SkyModel(comps=comps, modType=1)
+    if kind == "gaussspec":
+        coef = np.stack([rng.normal(-0.7, 0.3, k), rng.normal(0.0, 0.1, k)], axis=-1)
+        comps = np.concatenate([base, _gauss_cols(k, rng), coef], axis=1).astype(np.float32)
+        return synth.SkyModel(comps=comps, modType=11, nSpecTerm=2, imgRefFreq=float(uv.freqArr[128]))
+    alpha = rng.normal(-0.7, 0.3, k)
+    S = np.abs(base[:, :1]) * rng.uniform(0.5, 1.5, (k, 4))
+    comps = np.concatenate([base, _gauss_cols(k, rng), alpha[:, None], S], axis=1).astype(np.float32)
+    specFreq = np.array([uv.freqArr[i * 64 + 32] for i in range(4)])
+    specIndex = np.repeat(np.arange(4), 64).astype(np.int32)
+    return synth.SkyModel(comps=comps, modType=21, specFreq=specFreq, specIndex=specIndex)
+
+
+@pytest.mark.parametrize("kind", ["point", "gauss", "gaussspec", "gausstspec"])
+def test_buffer_component_parts(oracle, kind):
+    """Small per-buffer pieces cut the component list into parts whose FP64
+    sums k_dft_combine adds and applies: the model (replace) against the
+    FP64 oracle, and subtract / divide against the resident launch, for
+    point, Gaussian and spectral types; the last piece is ragged."""
+    from obit_amd.skymodel import SkyModelDFT
+    uv, sky0, _ = synth.workload(2, rows=1100, ncomp=2000)
+    sky = _split_sky(kind, uv, sky0, np.random.default_rng(31))
+    with SkyModelDFT(uv, sky, op="repl", nVisPIO=512) as sm:
+        out = sm.calc(uv.buffer.copy())
+        cpt, nsplit = sm.buffer_shape()
+    print(f"{kind}: {cpt} channels per thread, {nsplit} component parts", flush=True)
+    assert nsplit > 1
+    ref = oracle.dft_model_fp64(uv, sky)
+    peak = np.abs(ref).max()
+    err = np.abs(_model_from(uv, out) - ref).max() / peak
+    print(f"{kind} (replace): max {err:.3e}", flush=True)
+    assert err <= TOL
+    assert np.array_equal(out[:, :uv.nrparm], uv.buffer[:, :uv.nrparm])
+    for op in ("sub", "div"):
+        a = _run(uv, sky, op=op, path="buffer", nVisPIO=512)
+        b = _run(uv, sky, op=op, path="resident")
+        da = a[:, uv.nrparm:].reshape(uv.nrow, -1, 3)
+        db = b[:, uv.nrparm:].reshape(uv.nrow, -1, 3)
+        if op == "sub":
+            assert np.abs(da[..., :2] - db[..., :2]).max() <= TOL * peak
+            assert np.array_equal(da[..., 2], db[..., 2])
+        else:
+            # where the model is not small the quotient is well conditioned
+            m = np.repeat(np.abs(ref).reshape(uv.nrow, -1), uv.nstok, axis=1) > 0.1 * peak
+            rel = np.abs(da[..., :2] - db[..., :2]) / (np.abs(db[..., :2]).max() + 1e-30)
+            assert rel[m].max() <= 1e-4
+            assert np.allclose(da[..., 2][m], db[..., 2][m], rtol=1e-4)
