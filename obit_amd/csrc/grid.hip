@@ -732,15 +732,18 @@ constexpr int kColStride = N + N / 8 + 1;
 // the length of the transforms already formed and k = j mod Ns, the R
 // points j + r N / R times W_{Ns R}^{r k}, a radix-R DFT, written to
 // (j / Ns) Ns R + k + r Ns.
-template <int N, int R, int CB>
-__device__ __forceinline__ void stockham_pass(float2 *lds, int Ns, const float2 *__restrict__ tw) {
+// With HALF, tw holds W_N^m for m < N / 2 only (W_N^{m + N/2} = -W_N^m).
+// tid is the thread's index in the block.
+template <int N, int R, int CB, int NT = 256, bool HALF = false>
+__device__ __forceinline__ void stockham_pass(float2 *lds, int Ns, const float2 *__restrict__ tw,
+                                              int tid = (int)threadIdx.x) {
   constexpr int kItems = CB * N / R;
-  constexpr int kPer = (kItems + 255) / 256;
+  constexpr int kPer = (kItems + NT - 1) / NT;
   float2 v[kPer][R];
   int dcol[kPer], dn[kPer];
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
-    const int it = q * 256 + (int)threadIdx.x;
+    const int it = q * NT + tid;
     dcol[q] = -1;
     if (it < kItems) {
       const int c = it / (N / R), j = it - c * (N / R);
@@ -750,7 +753,14 @@ __device__ __forceinline__ void stockham_pass(float2 *lds, int Ns, const float2 
       for (int r = 0; r < R; ++r) {
         float2 x = lds[c * kColStride<N> + lidx(j + r * (N / R))];
         if (r) {
-          const float2 w = tw[(r * k * step) & (N - 1)];
+          const int m = (r * k * step) & (N - 1);
+          float2 w;
+          if (HALF) {
+            w = tw[m & (N / 2 - 1)];
+            if (m & (N / 2)) w = make_float2(-w.x, -w.y);
+          } else {
+            w = tw[m];
+          }
           x = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
         }
         v[q][r] = x;
@@ -771,58 +781,108 @@ __device__ __forceinline__ void stockham_pass(float2 *lds, int Ns, const float2 
 }
 
 // In-place backward (exp +i) complex transform of length N along v of
-// every column of a facet's padded half planes (all pitchC = lrow / 2
-// columns of a row, the hw padding columns left of u = 0 included, so
-// blocks of 4 columns are 16-byte aligned): block (column block, plane).
-// Thread t reads rows t + (N / R1) r, r < R1, of the block's 4 columns
-// (two 16-byte loads a row) and runs the first radix-R1 pass on them in
-// registers; the two other Stockham passes (N = R1 R2 R3) go through LDS;
-// the block is written back row by row.  tw[m] = exp(+2 pi i m / N),
-// formed in FP64 on the host.
-// Blocks are dispatched to the 8 XCDs in turn, each with its own L2; block
-// x works on column block (x mod 8) * perXcd + x / 8, so an XCD takes
-// neighbouring column blocks one after another and its L2 serves the rest
-// of each 128-byte row segment that a neighbour has fetched.
+// columns of a facet's padded half planes: thread t of a block reads rows
+// t + (N / R1) r, r < R1, of its columns and runs the first radix-R1 pass
+// on them in registers; the two other Stockham passes (N = R1 R2 R3) go
+// through LDS; the columns are written back in place.  tw[m] = exp(+2 pi
+// i m / N), formed in FP64 on the host.
+//
+// The same transform, 8 columns at a time by persistent blocks of 512
+// threads, so each row's 64-byte segment is read and written whole by one
+// wave: wave w, lane L takes rows q + 256 r (q = 32 w + L mod 32) and,
+// with h = L / 32, columns 2h, 2h+1 (bytes 16h..16h+15 of the segment) and
+// 4+2h, 5+2h, so each 16-byte access instruction covers 32 contiguous
+// bytes of 32 rows.  The work is the (plane, 8-column block) list of
+// columns c0 .. c0 + 8 nfull - 1 (c0 even) of nplane planes: the blocks of
+// XCD x (blockIdx mod 8; gridDim.x a multiple of 8) take its x-th eighth,
+// so an XCD works through neighbouring column blocks.  A block loads its
+// next column block into registers as soon as the current one's first
+// pass is in LDS, so the loads run under the LDS passes and the write-back.  The half twiddle table is copied to LDS, so the passes make
+// no global loads that would wait behind those.  Then the blocks take the
+// column side of every plane, one plane at a time.
 template <int N, int R1, int R2, int R3>
-__global__ __launch_bounds__(256) void k_fft_cols(float2 *grid, int pitchC, long long planeC, int nblk,
-                                                  const float2 *__restrict__ tw) {
-  constexpr int CB = 4;
+__global__ __launch_bounds__(512) void k_fft_cols8(float2 *grid, int pitchC, long long planeC, int c0, int nplane,
+                                                   int nfull, int side, const float2 *__restrict__ tw) {
+  constexpr int CB = 8, NT = 512;
   static_assert(R1 * R2 * R3 == N && N / R1 == 256, "radices: the first pass has one item a thread");
   __shared__ float2 lds[CB * kColStride<N>];
+  __shared__ float2 twl[N / 2];
   const int t = threadIdx.x;
-  const int perXcd = (nblk + 7) / 8;
-  const int blk = (int)(blockIdx.x & 7) * perXcd + (int)(blockIdx.x >> 3);
-  if (blk >= nblk) return;
-  float2 *g = grid + (long long)blockIdx.y * planeC + (long long)blk * CB;
-  float2 v[CB][R1];
+  for (int i = t; i < N / 2; i += NT) twl[i] = tw[i];
+  const int total = nplane * nfull, span = (total + 7) / 8, nper = (int)(gridDim.x >> 3);
+  const int x = (int)(blockIdx.x & 7);
+  const int eEnd = min(total, (x + 1) * span);
+  auto base = [&](int e) {
+    const int p = e / nfull, blk = e - p * nfull;
+    return grid + (long long)p * planeC + c0 + (long long)blk * CB;
+  };
+  float2 v[4][R1];
+  // rows r0 <= r < r1 of column block e into v, for thread tl
+  auto load = [&](int e, int r0, int r1, int tl) {
+    const float2 *g = base(e);
+    const int h = (tl >> 5) & 1, q = (tl >> 6) * 32 + (tl & 31);
 #pragma unroll
-  for (int r = 0; r < R1; ++r) {
-    const float4 *src = reinterpret_cast<const float4 *>(g + (long long)(t + 256 * r) * pitchC);
-    const float4 a = src[0], b = src[1];
-    v[0][r] = make_float2(a.x, a.y);
-    v[1][r] = make_float2(a.z, a.w);
-    v[2][r] = make_float2(b.x, b.y);
-    v[3][r] = make_float2(b.z, b.w);
+    for (int r = r0; r < r1; ++r) {
+      const float2 *row = g + (long long)(q + 256 * r) * pitchC;
+      const float4 a = *reinterpret_cast<const float4 *>(row + 2 * h);
+      const float4 b = *reinterpret_cast<const float4 *>(row + 4 + 2 * h);
+      v[0][r] = make_float2(a.x, a.y);
+      v[1][r] = make_float2(a.z, a.w);
+      v[2][r] = make_float2(b.x, b.y);
+      v[3][r] = make_float2(b.z, b.w);
+    }
+  };
+  int e = x * span + (int)(blockIdx.x >> 3);
+  if (e < eEnd) load(e, 0, R1, t);
+  for (; e < eEnd; e += nper) {
+    // the thread index through an empty asm, so the compiler forms the
+    // addresses and twiddle indices in each iteration instead of keeping
+    // them all in registers across the loop
+    int tl = t;
+    asm volatile("" : "+v"(tl));
+    const int h = (tl >> 5) & 1, q = (tl >> 6) * 32 + (tl & 31);
+    const int cid[4] = {2 * h, 2 * h + 1, 4 + 2 * h, 5 + 2 * h};
+    // the first pass (transforms of length 1 so far: no twiddles)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dft_small<R1>(v[c]);
+    __syncthreads();  // the last column block's write-back has read the LDS
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) lds[cid[c] * kColStride<N> + lidx(q * R1 + r)] = v[c][r];
+    // the next column block's rows, in flight under the passes
+    if (e + nper < eEnd) load(e + nper, 0, R1, tl);
+    __syncthreads();
+    stockham_pass<N, R2, CB, NT, true>(lds, R1, twl, tl);
+    stockham_pass<N, R3, CB, NT, true>(lds, R1 * R2, twl, tl);
+    float2 *g = base(e);
+#pragma unroll
+    for (int i = 0; i < N / 256; ++i) {
+      const int row = q + 256 * i;
+      float2 *dst = g + (long long)row * pitchC;
+      const int l = lidx(row);
+      const float2 x0 = lds[cid[0] * kColStride<N> + l], x1 = lds[cid[1] * kColStride<N> + l];
+      const float2 x2 = lds[cid[2] * kColStride<N> + l], x3 = lds[cid[3] * kColStride<N> + l];
+      *reinterpret_cast<float4 *>(dst + 2 * h) = make_float4(x0.x, x0.y, x1.x, x1.y);
+      *reinterpret_cast<float4 *>(dst + 4 + 2 * h) = make_float4(x2.x, x2.y, x3.x, x3.y);
+    }
   }
-  // the first pass (transforms of length 1 so far: no twiddles)
+  // column side of plane sp: threads t < 256 hold rows t + 256 r
+  for (int sp = (int)blockIdx.x; sp < nplane; sp += (int)gridDim.x) {
+    float2 *g = grid + (long long)sp * planeC + side;
+    __syncthreads();
+    if (t < 256) {
+      float2 w[R1];
 #pragma unroll
-  for (int c = 0; c < CB; ++c) {
-    dft_small<R1>(v[c]);
+      for (int r = 0; r < R1; ++r) w[r] = g[(long long)(t + 256 * r) * pitchC];
+      dft_small<R1>(w);
 #pragma unroll
-    for (int r = 0; r < R1; ++r) lds[c * kColStride<N> + lidx(t * R1 + r)] = v[c][r];
-  }
-  __syncthreads();
-  stockham_pass<N, R2, CB>(lds, R1, tw);
-  stockham_pass<N, R3, CB>(lds, R1 * R2, tw);
-#pragma unroll
-  for (int i = 0; i < N / 256; ++i) {
-    const int row = t + 256 * i;
-    float4 *dst = reinterpret_cast<float4 *>(g + (long long)row * pitchC);
-    const int l = lidx(row);
-    const float2 x0 = lds[l], x1 = lds[kColStride<N> + l], x2 = lds[2 * kColStride<N> + l],
-                 x3 = lds[3 * kColStride<N> + l];
-    dst[0] = make_float4(x0.x, x0.y, x1.x, x1.y);
-    dst[1] = make_float4(x2.x, x2.y, x3.x, x3.y);
+      for (int r = 0; r < R1; ++r) lds[lidx(t * R1 + r)] = w[r];
+    }
+    __syncthreads();
+    stockham_pass<N, R2, 1, NT, true>(lds, R1, twl);
+    stockham_pass<N, R3, 1, NT, true>(lds, R1 * R2, twl);
+    for (int i = t; i < N; i += NT) g[(long long)i * pitchC] = lds[lidx(i)];
   }
 }
 
@@ -837,14 +897,14 @@ __global__ __launch_bounds__(256) void k_fft_cols(float2 *grid, int pitchC, long
 // Output row y of plane p is input row r = (y + ny/2) mod ny times
 // (-1)^r xc[y] invNorm[p], and column x is sample (x + N/2) mod N times
 // yc[x] (ObitFArray2DCenter and ObitFArrayMulColRow, as k_center_correct4
-// applies them).  Block (4 rows, plane); 256 threads.  twN[k] = exp(+2 pi
+// applies them).  Block (RW rows, plane); NT threads.  twN[k] = exp(+2 pi
 // i k / N) and twM = the M-point table (every other entry of twN).
-template <int N, int R1, int R2, int R3>
-__global__ __launch_bounds__(256) void k_fft_rows_c2r(const float2 *grid, int pitchC, long long planeC, int hw,
+template <int N, int R1, int R2, int R3, int RW = 4, int NT = 256>
+__global__ __launch_bounds__(NT) void k_fft_rows_c2r(const float2 *grid, int pitchC, long long planeC, int hw,
                                                       int ny, const float2 *__restrict__ twN,
                                                       const float2 *__restrict__ twM, const float *xc,
                                                       const float *yc, const float *invNorm, float *img) {
-  constexpr int M = N / 2, RW = 4;
+  constexpr int M = N / 2;
   static_assert(R1 * R2 * R3 == M, "radices");
   __shared__ float2 lds[RW * kColStride<M>];
   const int t = threadIdx.x, p = blockIdx.y;
@@ -852,28 +912,28 @@ __global__ __launch_bounds__(256) void k_fft_rows_c2r(const float2 *grid, int pi
   const float2 *g = grid + (long long)p * planeC + hw;
   // rows r0 .. r0+3, samples 0 .. M (row c at lds[c * kColStride<M>]):
   // every load issued before the first is waited on
-  static_assert(M % 256 == 0, "whole sample blocks per thread");
-  constexpr int kPerRow = M / 256;
+  static_assert(M % NT == 0 && NT >= RW, "whole sample blocks per thread");
+  constexpr int kPerRow = M / NT;
   float2 xin[RW][kPerRow];
   float2 xlast = make_float2(0.0f, 0.0f);
 #pragma unroll
   for (int c = 0; c < RW; ++c)
 #pragma unroll
-    for (int i = 0; i < kPerRow; ++i) xin[c][i] = g[(long long)(r0 + c) * pitchC + t + 256 * i];
+    for (int i = 0; i < kPerRow; ++i) xin[c][i] = g[(long long)(r0 + c) * pitchC + t + NT * i];
   if (t < RW) xlast = g[(long long)(r0 + t) * pitchC + M];
 #pragma unroll
   for (int c = 0; c < RW; ++c)
 #pragma unroll
-    for (int i = 0; i < kPerRow; ++i) lds[c * kColStride<M> + lidx(t + 256 * i)] = xin[c][i];
+    for (int i = 0; i < kPerRow; ++i) lds[c * kColStride<M> + lidx(t + NT * i)] = xin[c][i];
   if (t < RW) lds[t * kColStride<M> + lidx(M)] = xlast;
   __syncthreads();
   // Z in place: the pair (k, M - k) by one thread; all of a thread's
   // loads first
-  constexpr int kPairs = RW * (M / 2 + 1), kPP = (kPairs + 255) / 256;
+  constexpr int kPairs = RW * (M / 2 + 1), kPP = (kPairs + NT - 1) / NT;
   float2 za[kPP], zb[kPP], wa[kPP], wb[kPP];
 #pragma unroll
   for (int i = 0; i < kPP; ++i) {
-    const int q = t + 256 * i;
+    const int q = t + NT * i;
     const int c = q / (M / 2 + 1), k = q - c * (M / 2 + 1);
     if (q < kPairs) {
       const float2 *row = lds + c * kColStride<M>;
@@ -886,7 +946,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_c2r(const float2 *grid, int pi
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPP; ++i) {
-    const int q = t + 256 * i;
+    const int q = t + NT * i;
     const int c = q / (M / 2 + 1), k = q - c * (M / 2 + 1);
     if (q < kPairs) {
       float2 *row = lds + c * kColStride<M>;
@@ -904,14 +964,14 @@ __global__ __launch_bounds__(256) void k_fft_rows_c2r(const float2 *grid, int pi
     }
   }
   __syncthreads();
-  stockham_pass<M, R1, RW>(lds, 1, twM);
-  stockham_pass<M, R2, RW>(lds, R1, twM);
-  stockham_pass<M, R3, RW>(lds, R1 * R2, twM);
+  stockham_pass<M, R1, RW, NT>(lds, 1, twM);
+  stockham_pass<M, R2, RW, NT>(lds, R1, twM);
+  stockham_pass<M, R3, RW, NT>(lds, R1 * R2, twM);
   // centre, correct, store: thread takes output column pairs
   const float inv = invNorm[p];
 #pragma unroll
-  for (int i = 0; i < RW * M / 256; ++i) {
-    const int q = t + 256 * i;
+  for (int i = 0; i < RW * M / NT; ++i) {
+    const int q = t + NT * i;
     const int c = q / M, m = q - c * M;
     const int r = r0 + c;
     int y = r + ny / 2;
@@ -929,9 +989,10 @@ __global__ __launch_bounds__(256) void k_fft_rows_c2r(const float2 *grid, int pi
 // image's (0, 0) sample is the sum of row 0's Hermitian extension,
 // X[0] + X[M] + 2 sum_{0<k<M} Re X[k] (imaginary parts of X[0], X[M]
 // dropped as the C2R drops them), times xc[ny/2] yc[nx/2]; one block a plane.
+// The norm also goes straight to hostNorm (coherent pinned memory).
 __global__ __launch_bounds__(256) void k_beam_norm_rows(const float2 *grid, long long planeC, int hw, int nx,
                                                         const float *xc, const float *yc, int ny, float *norm,
-                                                        float *invNorm) {
+                                                        float *invNorm, float *hostNorm) {
   __shared__ double s[256];
   const int p = blockIdx.x, M = nx / 2;
   const float2 *row = grid + (long long)p * planeC + hw;
@@ -947,6 +1008,7 @@ __global__ __launch_bounds__(256) void k_beam_norm_rows(const float2 *grid, long
     const float v = (float)s[0] * xc[ny / 2] * yc[nx / 2];
     norm[p] = v;
     invNorm[p] = (float)(1.0 / fmax(1.0e-20, (double)v));
+    hostNorm[p] = v;
   }
 }
 
@@ -954,12 +1016,13 @@ __global__ __launch_bounds__(256) void k_beam_norm_rows(const float2 *grid, long
 // centred image, (-1)^0 out[p][0][0] x xCorr[ny/2] x yCorr[nx/2]; norm and
 // its reciprocal (as the reference forms it, 1 / max(1e-20, norm)).
 __global__ void k_beam_norm(const float *fft, int nx, int ny, int nplane, const float *xc, const float *yc,
-                            float *norm, float *invNorm) {
+                            float *norm, float *invNorm, float *hostNorm) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nplane) return;
   const float v = fft[(size_t)p * ny * nx] * xc[ny / 2] * yc[nx / 2];
   norm[p] = v;
   invNorm[p] = (float)(1.0 / fmax(1.0e-20, (double)v));
+  hostNorm[p] = v;
 }
 
 // Per-sample outputs of prep_sample for the parity tests.
@@ -1019,7 +1082,8 @@ struct GpuState {
   float *h_norm = nullptr;  // pinned: plane norms / reciprocals staged for the caller
   float *d_fft = nullptr;
   size_t fftCap = 0;
-  float2 *d_tw = nullptr;          // exp(+2 pi i m / twN) for k_fft_cols
+  float2 *d_tw = nullptr;          // exp(+2 pi i m / twN) for the FFT kernels
+  int colsPerCU = 0, colsPerCU1024 = 0;  // resident k_fft_cols8 blocks per CU
   int twN = 0;
   float *d_corr = nullptr;  // xCorr[nx], yCorr[ny], 1/norm[nplane]
   size_t corrCap = 0;
@@ -1727,7 +1791,8 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
     gs->corrNx = nx;
     gs->corrNy = ny;
   }
-  if (!gs->h_norm) OBIT_HIP_TRY(hipHostMalloc(&gs->h_norm, sizeof(float) * 2 * 1024, hipHostMallocDefault));
+  // coherent, so the norm kernels' stores reach it without a copy
+  if (!gs->h_norm) OBIT_HIP_TRY(hipHostMalloc(&gs->h_norm, sizeof(float) * 2 * 1024, hipHostMallocCoherent));
   float *d_norm = gs->d_corr + nx + ny, *d_inv = d_norm + 1024;
   const float *xc = gs->d_corr, *yc = gs->d_corr + nx;
   if (!isBeam) {
@@ -1740,8 +1805,9 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
   // planes (row pitch lrow/2 complex, the u = 0 column hw complex into each
   // row, planes sizeGrid floats apart), then ObitFArray2DCenter's quadrant
   // swap and ObitFArrayMulColRow's correction.  For a side of 1024 or 2048:
-  // k_fft_cols transforms every column along v in place (the grid is
-  // consumed), a beam's norm is summed from row 0 of the result, and
+  // k_fft_cols8 transforms the columns u = 0 .. nx/2 along v
+  // in place (the grid is consumed), a beam's norm is summed from row 0 of
+  // the result, and
   // k_fft_rows_c2r transforms every row along u, centres, corrects and
   // writes the image: two passes over the planes.  Other sizes: one batched
   // 2-D C2R (hipFFT) into scratch, then k_center_correct4.
@@ -1766,25 +1832,41 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
     const float2 *twN = gs->d_tw, *twM = gs->d_tw + ny;
     float2 *g2 = reinterpret_cast<float2 *>(f.grid);
     const long long planeC = (long long)f.sizeGrid / 2;
-    // every column of the padded rows, 4 at a time (8 x ceil(blocks / 8)
-    // launched; the spare ones return at once)
-    const int nblk = pitchC / 4;
-    const dim3 gcol((unsigned)(8 * ((nblk + 7) / 8)), (unsigned)nplane);
-    const dim3 grow((unsigned)(ny / 4), (unsigned)nplane);
+    // The nx/2 + 1 columns u = 0 .. nx/2 (hw .. hw + nx/2 of the padded
+    // rows; the hw padding columns left of them are not read again): nx/2
+    // of them in 8-column blocks starting at an even column, so the list
+    // of nplane x nx/16 blocks divides evenly over the CUs, and the one
+    // left over (u = 0 or u = nx/2) per plane after them.
+    const int c0 = (f.hw % 2) ? f.hw + 1 : f.hw;
+    const int side = (f.hw % 2) ? f.hw : f.hw + nx / 2;
+    const int nfull = nx / 16;
+    if (gs->colsPerCU <= 0) {
+      int nb = 0;
+      OBIT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, obit::grid::k_fft_cols8<2048, 8, 16, 16>, 512, 0));
+      gs->colsPerCU = std::max(nb, 1);
+      OBIT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, obit::grid::k_fft_cols8<1024, 4, 16, 16>, 512, 0));
+      gs->colsPerCU1024 = std::max(nb, 1);
+    }
+    const long long resident = (long long)gs->nCU * (nx == 2048 ? gs->colsPerCU : gs->colsPerCU1024);
+    const long long ncb = 8 * ((std::min<long long>((long long)nplane * nfull, resident) + 7) / 8);
+    const dim3 gcol((unsigned)ncb);
+    // rows a block: 1 at 2048 (measured 8.1 ms for 320 planes, against 8.2
+    // with 2 rows, 8.4 with 4 and 9.35 with 8), 4 at 1024
+    const dim3 grow((unsigned)(ny / 4), (unsigned)nplane), grow2048((unsigned)ny, (unsigned)nplane);
     if (nx == 2048)
-      hipLaunchKernelGGL((obit::grid::k_fft_cols<2048, 8, 16, 16>), gcol, dim3(256), 0, gs->stream, g2, pitchC,
-                         planeC, nblk, twN);
+      hipLaunchKernelGGL((obit::grid::k_fft_cols8<2048, 8, 16, 16>), gcol, dim3(512), 0, gs->stream, g2, pitchC,
+                         planeC, c0, nplane, nfull, side, twN);
     else
-      hipLaunchKernelGGL((obit::grid::k_fft_cols<1024, 4, 16, 16>), gcol, dim3(256), 0, gs->stream, g2, pitchC,
-                         planeC, nblk, twN);
+      hipLaunchKernelGGL((obit::grid::k_fft_cols8<1024, 4, 16, 16>), gcol, dim3(512), 0, gs->stream, g2, pitchC,
+                         planeC, c0, nplane, nfull, side, twN);
     OBIT_HIP_TRY(hipGetLastError());
     if (isBeam) {
       hipLaunchKernelGGL(obit::grid::k_beam_norm_rows, dim3((unsigned)nplane), dim3(256), 0, gs->stream,
-                         (const float2 *)g2, planeC, f.hw, nx, xc, yc, ny, d_norm, d_inv);
+                         (const float2 *)g2, planeC, f.hw, nx, xc, yc, ny, d_norm, d_inv, gs->h_norm);
       OBIT_HIP_TRY(hipGetLastError());
     }
     if (nx == 2048)
-      hipLaunchKernelGGL((obit::grid::k_fft_rows_c2r<2048, 4, 16, 16>), grow, dim3(256), 0, gs->stream,
+      hipLaunchKernelGGL((obit::grid::k_fft_rows_c2r<2048, 4, 16, 16, 1>), grow2048, dim3(256), 0, gs->stream,
                          (const float2 *)g2, pitchC, planeC, f.hw, ny, twN, twM, xc, yc, (const float *)d_inv,
                          d_image);
     else
@@ -1823,7 +1905,7 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
     if (r != HIPFFT_SUCCESS) return obit::fail(obit::kHipError, "hipfftExecC2R: %d", (int)r);
     if (isBeam) {
       hipLaunchKernelGGL(obit::grid::k_beam_norm, dim3((nplane + 63) / 64), dim3(64), 0, gs->stream, gs->d_fft, nx,
-                         ny, nplane, xc, yc, d_norm, d_inv);
+                         ny, nplane, xc, yc, d_norm, d_inv, gs->h_norm);
       OBIT_HIP_TRY(hipGetLastError());
     }
     dim3 grid((unsigned)((nx / 4 + 255) / 256), (unsigned)ny, (unsigned)nplane);
@@ -1831,8 +1913,6 @@ int ObitHIPGridFFT2Image(CUDAGridInfo *in, long ifacet, int isBeam, float *beamN
                        d_inv, d_image);
     OBIT_HIP_TRY(hipGetLastError());
   }
-  if (isBeam)
-    OBIT_HIP_TRY(hipMemcpyAsync(gs->h_norm, d_norm, sizeof(float) * nplane, hipMemcpyDeviceToHost, gs->stream));
   OBIT_HIP_TRY(hipStreamSynchronize(gs->stream));
   if (isBeam) {
     for (int p = 0; p < nplane; ++p) {
