@@ -359,6 +359,18 @@ static_assert(kS >= kTC && (4 * kS) % 64 == 48, "tile stride: room for the halo,
 static_assert(kPlane % 64 == 32 && kPlane >= kTR * kS, "im plane offset must be 32 mod 64");
 static_assert(kPlane % 4 == 0, "planes are cleared as float4");
 
+// a in the lanes whose bit is set in the wave-uniform mask m, else 0: one
+// select on a scalar lane mask.  The operand column of lane l is l / 16,
+// so samples k0 <= k < k1 of a group are lanes 16 k0 .. 16 k1 - 1.  The
+// result feeds a matrix step, which may read a VGPR only 2 wait states
+// after a VALU instruction wrote it; the compiler does not see into the
+// asm, so the asm waits them itself.
+__device__ __forceinline__ float keep_lanes(float a, unsigned long long m) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2\n\ts_nop 1" : "=v"(r) : "v"(a), "s"(m));
+  return r;
+}
+
 // Work items are (tile list, chunk of <= kChunk tracks).  Each wave takes
 // its own item and owns a private (TV + 6) x (TU + 8) tile (separate re /
 // im planes) in LDS.
@@ -420,6 +432,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
   float *im = re + kPlane;
   int2 *si = s_si[wv];
   float2 *sv = s_sv[wv];
+  // this lane's part of a staged visibility: sample mk of a group, real
+  // part for window rows (mm < 8), imaginary for the rest
+  const float *svl = reinterpret_cast<const float *>(sv) + 2 * mk + (mm >= 8 ? 1 : 0);
   int *scell = s_cell[wv];
   for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
        w += (long long)gridDim.x * kWaves) {
@@ -538,9 +553,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
 #pragma unroll
       for (int g = 0; g < kGrp; ++g) {
         const int2 q = si[4 * g + mk];
-        const float2 x = sv[4 * g + mk];
         B[g] = gcf[q.x + mm];
-        A[g] = gcf[q.y + mm7] * (mm < 8 ? x.x : x.y);
+        A[g] = gcf[q.y + mm7] * svl[8 * g];
       }
 #pragma unroll
       for (int g = 0; g < kGrp; ++g) {
@@ -550,9 +564,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
         while (mg) {
           const int p1 = __builtin_ctz(mg);
           if (p1 > s0) {
-            const unsigned seg = (1u << p1) - (1u << s0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g], acc,
-                                                       0, 0, 0);
+            // samples s0 .. p1 - 1 of the group (p1 <= 3)
+            const unsigned long long m = (~0ull << (16 * s0)) ^ (~0ull << (16 * p1));
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(keep_lanes(A[g], m), B[g], acc, 0, 0, 0);
           }
           handover();
           cur = __builtin_amdgcn_readlane(cellq, 4 * g + p1);
@@ -560,9 +574,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
           s0 = p1;
           mg &= mg - 1;
         }
-        const unsigned seg = 16u - (1u << s0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((seg >> mk) & 1u) ? A[g] : 0.0f, B[g], acc, 0,
-                                                   0, 0);
+        // samples s0 .. 3 of the group
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(keep_lanes(A[g], ~0ull << (16 * s0)), B[g], acc, 0, 0, 0);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
