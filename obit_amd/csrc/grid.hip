@@ -75,6 +75,8 @@ struct Common {
   const int *pcount;    // [nplane]
   const float *pfmin, *pfmax;  // plane frequency extremes
   int maxpc;
+  const float *pfreq;   // FP32 frequency factor of each plane's channels [nplane][maxpc]
+  const int *pmono;     // [nplane]: 1 if pfreq is monotone along the plane's channel list
   const float *gcf;     // [101][8]: row it = taps 0..6 of fractional offset it, then 0
 };
 
@@ -238,34 +240,84 @@ __global__ void k_uvw_hash(const float *vis, long long nrow, int lenvis, unsigne
   if ((threadIdx.x & 63) == 0 && h) atomicAdd(out, h);
 }
 
-// Every (plane, tile of base cells) a track can reach, with the window
-// block of the tile it enters first: fn(L, wkey).  A track's base cells
-// are bounded by the cells at the plane's lowest and highest frequency
-// (cells are monotonic in frequency); it enters a tile near the cell of
-// the plane's first channel, which clamped into the tile gives the 8x2
-// block key (wkey = block row * 8 + block column, < 80).
+// Every (plane, tile of base cells) a track's samples have base cells in,
+// with the channels that do and the window block where they enter:
+// fn(L, wkey, lo, hi) for channels lo <= i < hi of the plane's channel list.
+// A track's base cells are bounded by the cells at the plane's lowest and
+// highest frequency (cells are monotonic in frequency), which gives the
+// candidate tiles.  Where the plane's frequencies are monotonic along its
+// channel list, so are the cells (the same FP32 arithmetic as prep_cells,
+// each operation monotonic), and a binary search over the list finds
+// exactly the channels whose base cell lies in the tile; a tile no channel
+// lands in is left out.  Other planes give every channel to every
+// candidate tile.  wkey is the 8x2 block of the tile holding the cell of
+// the first channel given (clamped into the tile; block row * 8 + block
+// column, < 80).
 template <typename Fn>
 __device__ __forceinline__ void track_tiles(const FacetDev &f, const Common &cm, float uu, float vv,
                                             Fn fn) {
   const int ncol = f.lrow / 2;
   for (int p = 0; p < cm.nplane; ++p) {
-    if (cm.pcount[p] <= 0) continue;
-    int iu0, iv0, iu1, iv1, iuf, ivf;
+    const int npc = cm.pcount[p];
+    if (npc <= 0) continue;
+    int iu0, iv0, iu1, iv1;
     cells_at(f, uu, vv, cm.pfmin[p], iu0, iv0);
     cells_at(f, uu, vv, cm.pfmax[p], iu1, iv1);
-    cells_at(f, uu, vv, cm.freq[cm.pchan[(size_t)p * cm.maxpc]], iuf, ivf);
     int cA = min(iu0, iu1), cB = max(iu0, iu1);
     int rA = min(iv0, iv1), rB = max(iv0, iv1);
     cA = max(cA, 0); rA = max(rA, 0);
     cB = min(cB, ncol - 1); rB = min(rB, f.ny - 1);
     if (cA > cB || rA > rB) continue;
+    const float *pf = cm.pfreq + (size_t)p * cm.maxpc;
+    const bool mono = cm.pmono[p] != 0;
+    // cells of the first and last channel of the list: the directions in
+    // which the cells move along it
+    int iuF, ivF, iuL, ivL;
+    cells_at(f, uu, vv, pf[0], iuF, ivF);
+    cells_at(f, uu, vv, pf[npc - 1], iuL, ivL);
     const int tu0 = cA / TU, tu1 = cB / TU, tv0 = rA / TV, tv1 = rB / TV;
     const long long base = f.listBase + (long long)p * f.ntu * f.ntv;
     for (int tv = tv0; tv <= tv1; ++tv)
       for (int tu = tu0; tu <= tu1; ++tu) {
-        const int cu = min(max(iuf - tu * TU, 0), TU - 1);
-        const int cv = min(max(ivf - tv * TV, 0), TV - 1);
-        fn(base + (long long)tv * f.ntu + tu, (unsigned)((cv >> 1) * (TU / 8) + (cu >> 3)));
+        const int C0 = tu * TU, C1 = C0 + TU, R0 = tv * TV, R1 = R0 + TV;
+        int lo = 0, hi = npc;
+        if (mono) {
+          // first index in [a, b) where pred holds (pred false, then true)
+          auto first = [&](auto pred) {
+            int a = 0, b = npc;
+            while (a < b) {
+              const int m = (a + b) >> 1;
+              int iu, iv;
+              cells_at(f, uu, vv, pf[m], iu, iv);
+              if (pred(iu, iv)) b = m; else a = m + 1;
+            }
+            return a;
+          };
+          if (min(iuF, iuL) < C0 || max(iuF, iuL) >= C1) {
+            if (iuF <= iuL) {
+              lo = max(lo, first([&](int iu, int) { return iu >= C0; }));
+              hi = min(hi, first([&](int iu, int) { return iu >= C1; }));
+            } else {
+              lo = max(lo, first([&](int iu, int) { return iu < C1; }));
+              hi = min(hi, first([&](int iu, int) { return iu < C0; }));
+            }
+          }
+          if (min(ivF, ivL) < R0 || max(ivF, ivL) >= R1) {
+            if (ivF <= ivL) {
+              lo = max(lo, first([&](int, int iv) { return iv >= R0; }));
+              hi = min(hi, first([&](int, int iv) { return iv >= R1; }));
+            } else {
+              lo = max(lo, first([&](int, int iv) { return iv < R1; }));
+              hi = min(hi, first([&](int, int iv) { return iv < R0; }));
+            }
+          }
+          if (lo >= hi) continue;
+        }
+        int iue, ive;
+        cells_at(f, uu, vv, pf[lo], iue, ive);
+        const int cu = min(max(iue - C0, 0), TU - 1);
+        const int cv = min(max(ive - R0, 0), TV - 1);
+        fn(base + (long long)tv * f.ntu + tu, (unsigned)((cv >> 1) * (TU / 8) + (cu >> 3)), lo, hi);
       }
   }
 }
@@ -283,17 +335,18 @@ __global__ void k_bin_count(const float *vis, long long nrow, const FacetDev *fa
     float uu, vv, ww;
     rotate_uvw(f, vis + r * (long long)cm.lenvis, uu, vv, ww);
     unsigned n = 0;
-    track_tiles(f, cm, uu, vv, [&](long long, unsigned) { ++n; });
+    track_tiles(f, cm, uu, vv, [&](long long, unsigned, int, int) { ++n; });
     cnt[it] = n;
   }
 }
 
 // Binning, pass 2: every entry as a sort key (list << 7 | window block)
-// and its row, written at the thread's scanned offset.  Sorting the pairs
-// by key gives each tile's list ordered by the window its tracks enter
-// first, then by row (the sort is stable and the emission ordered).
+// and its (row, lo | hi << 16) channel range, written at the thread's
+// scanned offset.  Sorting the pairs by key gives each tile's list ordered
+// by the window its tracks enter first, then by row (the sort is stable
+// and the emission ordered).
 __global__ void k_bin_emit(const float *vis, long long nrow, const FacetDev *facets, int nf, Common cm,
-                           const unsigned long long *off, unsigned *keys, int *rows) {
+                           const unsigned long long *off, unsigned *keys, int2 *rows) {
   const long long total = nrow * (long long)nf;
   for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
        it += (long long)gridDim.x * blockDim.x) {
@@ -303,9 +356,9 @@ __global__ void k_bin_emit(const float *vis, long long nrow, const FacetDev *fac
     float uu, vv, ww;
     rotate_uvw(f, vis + r * (long long)cm.lenvis, uu, vv, ww);
     unsigned long long o = off[it];
-    track_tiles(f, cm, uu, vv, [&](long long L, unsigned wkey) {
+    track_tiles(f, cm, uu, vv, [&](long long L, unsigned wkey, int lo, int hi) {
       keys[o] = ((unsigned)L << 7) | wkey;
-      rows[o] = (int)r;
+      rows[o] = make_int2((int)r, lo | (hi << 16));
       ++o;
     });
   }
@@ -406,12 +459,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
                                                            const int *listFacet, const int2 *items,
                                                            long long nitems, Common cm,
                                                            const int *counts, const long long *offsets,
-                                                           const int *lists,
+                                                           const int2 *lists,
                                                            unsigned long long *stats) {
   __shared__ float4 s_tile4[kWaves][2 * kPlane / 4];  // re plane, then im plane
   __shared__ int2 s_si[kWaves][kHalf];      // staged sample: u and v tap-row offsets in s_gcfp
   __shared__ float2 s_sv[kWaves][kHalf];    // staged sample: weighted visibility
   __shared__ int s_cell[kWaves][64];
+  __shared__ int s_flag[kWaves][64];        // entry starts within a window of slots
   // padded GCF rows: 7 zeros, the 7 taps of fractional offset it, 10 zeros,
   // so every window column or row reads its tap (or a zero) with no test
   __shared__ float s_gcfp[101 * kGcfRow];
@@ -436,6 +490,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
   // part for window rows (mm < 8), imaginary for the rest
   const float *svl = reinterpret_cast<const float *>(sv) + 2 * mk + (mm >= 8 ? 1 : 0);
   int *scell = s_cell[wv];
+  int *flag = s_flag[wv];
   for (long long w = (long long)blockIdx.x * kWaves + wv; w < nitems;
        w += (long long)gridDim.x * kWaves) {
     const int2 item = items[w];
@@ -455,7 +510,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
     const int tv = t / f.ntu, tu = t - tv * f.ntu;
     const int col0 = tu * TU, row0 = tv * TV;
     for (int i = lane; i < 2 * kPlane / 4; i += 64) s_tile4[wv][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const int *lst = lists + offsets[L];
+    const int2 *lst = lists + offsets[L];
     const int npc = cm.pcount[p];
     if (npc <= 0) continue;  // planes without channels have no tracks
     const int *pch = cm.pchan + (size_t)p * cm.maxpc;
@@ -475,15 +530,61 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       for (int r = 0; r < 4; ++r) pv[r] = re[pend + r * kS];
       pa = acc;
     };
-    // slot (track e, channel index c) of this lane in the current pass and
-    // the next; a pass advances every lane by 64 slots.  The row and
-    // channel of a slot are read two passes ahead and its data one pass
-    // ahead; slots past the item's end read its last track (always valid)
-    // so every pass issues the same loads.
-    const int dq = 64 / npc, dr = 64 - dq * npc;
-    int ea = e0 + lane / npc, ca = lane - (lane / npc) * npc;
-    int eb = ea + dq, cb = ca + dr;
-    if (cb >= npc) { cb -= npc; ++eb; }
+    // The item's entries (at most kChunk = 128: lane j holds entries j and
+    // 64 + j): row, first channel lo (in the plane's channel list) and
+    // channel count, and where its slots start, P (an exclusive scan of the
+    // counts).  Slot s of entry k is channel lo_k + s - P_k of the list;
+    // pass q takes slots 64 q .. 64 q + 63, lanes in order.
+    const int ne = e1 - e0;
+    int rowA = 0, rowB = 0, cntA = 0, cntB = 0, loA = 0, loB = 0;
+    if (lane < ne) {
+      const int2 x = lst[e0 + lane];
+      rowA = x.x; loA = x.y & 0xFFFF; cntA = (x.y >> 16) - loA;
+    }
+    if (64 + lane < ne) {
+      const int2 x = lst[e0 + 64 + lane];
+      rowB = x.x; loB = x.y & 0xFFFF; cntB = (x.y >> 16) - loB;
+    }
+    int incA = cntA, incB = cntB;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int ya = __shfl_up(incA, o), yb = __shfl_up(incB, o);
+      if (lane >= o) { incA += ya; incB += yb; }
+    }
+    incB += __builtin_amdgcn_readlane(incA, 63);
+    const int nslot = __builtin_amdgcn_readlane(incB, 63);
+    const int PA = incA - cntA, PB = incB - cntB;
+    const int baseA = loA - PA, baseB = loB - PB;
+    // Each lane's row, list index of its channel and liveness for the
+    // window of slots S .. S + 63.  jA is the entry holding slot S; lane j
+    // looks at entry jA + 1 + j, whose start (if in the window) it marks in
+    // flag, and lane l's entry is jA plus the starts at or before S + l.
+    // Slots past the item's end take its last entry (always valid) so
+    // every pass issues the same loads.
+    int jA = 0;
+    auto map = [&](int S, int &row, int &ci, bool &live) {
+      const int idx = jA + 1 + lane;
+      const int pa = __shfl(PA, idx & 63), pb = __shfl(PB, idx & 63);
+      const int pos = idx < ne ? (idx < 64 ? pa : pb) - S : 65;  // 1 .. ; 65: no entry
+      flag[lane] = 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (pos < 64) flag[pos] = 1;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const unsigned long long starts = __ballot(flag[lane] != 0);
+      const int k = min(jA + (int)__popcll(starts & (~0ull >> (63 - lane))), ne - 1);
+      const int ra = __shfl(rowA, k & 63), rb2 = __shfl(rowB, k & 63);
+      const int ba = __shfl(baseA, k & 63), bb = __shfl(baseB, k & 63);
+      row = k < 64 ? ra : rb2;
+      ci = min((k < 64 ? ba : bb) + S + lane, npc - 1);
+      live = S + lane < nslot;
+      // the entry holding slot S + 64: one more if an entry starts there
+      const int c = (int)__popcll(starts);
+      jA += c + (__builtin_amdgcn_readlane(pos, c) == 64 ? 1 : 0);
+    };
     float xu, xv, xw, xre, xim, xwt, xfs;
     int xch;
     auto fetch = [&](int r, int ch) {
@@ -494,17 +595,27 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
       const float *x = row + cm.nrparm + 3 * ch;
       xre = x[0]; xim = x[1]; xwt = x[2];
     };
-    fetch(lst[min(ea, e1 - 1)], pch[ca]);
-    int rb = lst[min(eb, e1 - 1)], chb = pch[cb];
-    const int npass = ((e1 - e0) * npc + 63) / 64;
+    // the row and channel of a slot are read two passes ahead and its data
+    // one pass ahead
+    bool liveCur, liveNext;
+    int rb, chb;
+    {
+      int r0, c0;
+      map(0, r0, c0, liveCur);
+      fetch(r0, pch[c0]);
+      map(64, rb, c0, liveNext);
+      chb = pch[c0];
+    }
+    const int npass = (nslot + 63) / 64;
     for (int pass = 0; pass < npass; ++pass) {
-      const bool live = ea < e1;
+      const bool live = liveCur;
       const float u0 = xu, v0 = xv, w0 = xw, re0 = xre, im0 = xim, wt0 = xwt, fs0 = xfs;
       const int ch0 = xch;
       // the slot after next: its row and channel; then the next pass's data
-      int ec = eb + dq, cc2 = cb + dr;
-      if (cc2 >= npc) { cc2 -= npc; ++ec; }
-      const int rc = lst[min(ec, e1 - 1)], chc = pch[cc2];
+      int rc, cic;
+      bool liveC;
+      map(64 * (pass + 2), rc, cic, liveC);
+      const int chc = pch[cic];
       fetch(rb, chb);
       bool keep = false;
       Sample sm;
@@ -515,7 +626,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grid_taps(const float *vis, con
         keep = (sm.vr != 0.0f || sm.vi != 0.0f) && sm.iu >= col0 && sm.iu < col0 + TU &&
                sm.iv >= row0 && sm.iv < row0 + TV;
       }
-      ea = eb; ca = cb; eb = ec; cb = cc2; rb = rc; chb = chc;
+      liveCur = liveNext; liveNext = liveC; rb = rc; chb = chc;
       const unsigned long long mask = __ballot(keep);
       if (mask == 0) continue;
       const int nv = __popcll(mask);
@@ -1068,7 +1179,8 @@ struct GpuState {
   std::vector<FacetDev> h_facets;
   int *d_listFacet = nullptr;
   long long nlist = 0;
-  int *d_counts = nullptr, *d_lists = nullptr;
+  int *d_counts = nullptr;
+  int2 *d_lists = nullptr;          // per list entry: row, lo | hi << 16 (channels of the plane list)
   long long *d_offsets = nullptr;  // first entry of every list (64-bit: passes may exceed 2^31 entries)
   long long *d_lend = nullptr;     // one past the last entry of every list
   long long listCap = 0;
@@ -1076,7 +1188,7 @@ struct GpuState {
   unsigned long long *d_toff = nullptr;  // their exclusive scan
   long long tcap = 0;
   unsigned *d_keys = nullptr, *d_keys2 = nullptr;  // sort keys, in and out
-  int *d_rows = nullptr;           // rows before the sort (d_lists after it)
+  int2 *d_rows = nullptr;          // (row, channel range) before the sort (d_lists after it)
   int2 *d_items = nullptr;
   long long itemCap = 0;
   long long *d_itemOff = nullptr;  // first work item of every list (exclusive scan of chunk counts)
@@ -1103,7 +1215,8 @@ struct GpuState {
   std::vector<float> h_corr;  // xCorr, yCorr of the last (nx, ny) (GridCorrFn)
   int corrNx = 0, corrNy = 0;
   // tables
-  float *d_gcf = nullptr, *d_freq = nullptr, *d_sig = nullptr, *d_pf = nullptr;
+  float *d_gcf = nullptr, *d_freq = nullptr, *d_sig = nullptr, *d_pf = nullptr, *d_pfreq = nullptr;
+  int *d_pmono = nullptr;
   int *d_pchan = nullptr, *d_pcount = nullptr;
   Common cm{};
   bool ready = false;
@@ -1230,7 +1343,9 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   }
   int maxpc = 1;
   for (int p = 0; p < nplane; ++p) maxpc = std::max(maxpc, (int)pch[p].size());
-  std::vector<int> hpch((size_t)nplane * maxpc, 0), hpcount(nplane, 0);
+  if (maxpc > 32767) return fail(kInvalid, "%d channels in one plane: at most 32767", maxpc);
+  std::vector<int> hpch((size_t)nplane * maxpc, 0), hpcount(nplane, 0), hmono(nplane, 0);
+  std::vector<float> hpfreq((size_t)nplane * maxpc, 0.0f);
   for (int p = 0; p < nplane; ++p) {
     hpcount[p] = (int)pch[p].size();
     float lo = 0.0f, hi = 0.0f;
@@ -1241,6 +1356,19 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
       if (k == 0 || fr > hi) hi = fr;
     }
     pf[p] = lo; pf[nplane + p] = hi;
+    // the plane's frequencies along its channel list, and whether they
+    // are monotonic there (then so are a track's cells)
+    bool up = true, down = true;
+    for (size_t k = 0; k < pch[p].size(); ++k) {
+      const float fr = gi->h_freqArr[pch[p][k]];
+      hpfreq[(size_t)p * maxpc + k] = fr;
+      if (k) {
+        const float prev = gi->h_freqArr[pch[p][k - 1]];
+        up = up && fr >= prev;
+        down = down && fr <= prev;
+      }
+    }
+    hmono[p] = (up || down) ? 1 : 0;
   }
   bool hasSigma = false;
   if (gi->h_sigma1)
@@ -1248,6 +1376,11 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   // (re)allocate device tables
   (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
   (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
+  (void)hipFree(gs->d_pfreq); (void)hipFree(gs->d_pmono);
+  OBIT_HIP_TRY(hipMalloc(&gs->d_pfreq, sizeof(float) * hpfreq.size()));
+  OBIT_HIP_TRY(hipMalloc(&gs->d_pmono, sizeof(int) * nplane));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_pfreq, hpfreq.data(), sizeof(float) * hpfreq.size(), hipMemcpyHostToDevice));
+  OBIT_HIP_TRY(hipMemcpy(gs->d_pmono, hmono.data(), sizeof(int) * nplane, hipMemcpyHostToDevice));
   OBIT_HIP_TRY(hipMalloc(&gs->d_gcf, sizeof(float) * kGcf8));
   OBIT_HIP_TRY(hipMalloc(&gs->d_freq, sizeof(float) * nchan));
   OBIT_HIP_TRY(hipMalloc(&gs->d_sig, sizeof(float) * 3 * nchan));
@@ -1282,6 +1415,8 @@ int refresh(CUDAGridInfo *in, long iGPU, GpuState *gs) {
   cm.pchan = gs->d_pchan; cm.pcount = gs->d_pcount;
   cm.pfmin = gs->d_pf; cm.pfmax = gs->d_pf + nplane;
   cm.maxpc = maxpc;
+  cm.pfreq = gs->d_pfreq;
+  cm.pmono = gs->d_pmono;
   cm.gcf = gs->d_gcf;
   // list buffers sized for the lists; sort keys hold the list index in
   // 25 bits above the 7-bit window block
@@ -1411,8 +1546,8 @@ int grid_device(GpuState *gs, const float *vis, long long nrow, bool reuse) {
     gs->d_lists = nullptr; gs->d_keys = nullptr; gs->d_keys2 = nullptr; gs->d_rows = nullptr;
     gs->listCap = 0;
     const size_t m = (size_t)std::max<long long>(total, 1);
-    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int) * m);
-    if (e == hipSuccess) e = hipMalloc(&gs->d_rows, sizeof(int) * m);
+    hipError_t e = hipMalloc(&gs->d_lists, sizeof(int2) * m);
+    if (e == hipSuccess) e = hipMalloc(&gs->d_rows, sizeof(int2) * m);
     if (e == hipSuccess) e = hipMalloc(&gs->d_keys, sizeof(unsigned) * m);
     if (e == hipSuccess) e = hipMalloc(&gs->d_keys2, sizeof(unsigned) * m);
     if (e != hipSuccess) return fail(kNoMemory, "track lists (%lld entries): %s", total, hipGetErrorString(e));
@@ -1666,6 +1801,7 @@ void ObitCUDAGridShutdown(CUDAGridInfo *cudaInfo, long iGPU) {
       if (fi && fi->d_grid) { (void)hipFree(fi->d_grid); fi->d_grid = nullptr; }
     }
     (void)hipFree(gs->d_gcf); (void)hipFree(gs->d_freq); (void)hipFree(gs->d_sig); (void)hipFree(gs->d_pf);
+    (void)hipFree(gs->d_pfreq); (void)hipFree(gs->d_pmono);
     (void)hipFree(gs->d_pchan); (void)hipFree(gs->d_pcount); (void)hipFree(gs->d_facets); (void)hipFree(gs->d_listFacet);
     (void)hipFree(gs->d_counts); (void)hipFree(gs->d_lend); (void)hipFree(gs->d_offsets); (void)hipFree(gs->d_lists);
     (void)hipFree(gs->d_tcnt); (void)hipFree(gs->d_toff); (void)hipFree(gs->d_keys); (void)hipFree(gs->d_keys2);
