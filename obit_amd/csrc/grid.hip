@@ -46,7 +46,10 @@ namespace obit {
 namespace grid {
 
 constexpr int TU = 64;        // tile width in grid columns (complex cells)
-constexpr int TV = 20;        // tile height in grid rows (LDS for 2 four-wave blocks per CU)
+// tile height in grid rows: the LDS of 3 four-wave blocks per CU (12 waves;
+// measured on workload 4: 50.0 ms a pass, against 58.3 at 20 rows and 58.9
+// at 12 with 8 waves, 59.9 at 6 rows)
+constexpr int TV = 8;
 constexpr int kGcf = 708;     // tabulated GCF, CPU gridding order
 constexpr int kGcf8 = 101 * 8; // the same, rows padded to 8 on the device
 
@@ -428,9 +431,10 @@ __device__ __forceinline__ float keep_lanes(float a, unsigned long long m) {
 // its own item and owns a private (TV + 6) x (TU + 8) tile (separate re /
 // im planes) in LDS.
 //
-// The item's (track, channel) samples are visited 64 at a time, lanes =
-// samples in track-major, channel-minor order; the rows for the next 64
-// are fetched while the current 64 are gridded.  Each sample goes through
+// The item's samples, each entry's channels whose base cells the binning
+// placed in this tile, are visited 64 at a time, lanes = samples in entry
+// order, channels in list order; the rows for the next 64 are fetched
+// while the current 64 are gridded.  Each sample goes through
 // the CPU gridder's arithmetic (prep_vals); those whose base cell lies in
 // the tile are kept, in order, a 64-bit mask marks the kept
 // samples whose window differs from the previous one's, and they pass
