@@ -267,6 +267,221 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
   }
 }
 
+// ---- the matrix-core DFT (point components) ----
+//
+// Channel c = c1 + 8 c2 (c1, c2 < 8) of a 64-channel group g (an IF's
+// chunk of 64 selected channels) is at frequency f_g(36) + df (c1 - 4) +
+// 8 df (c2 - 4), so a point component's term factors as
+//   a e(psi f) = X[c1] Y_g[c2],  X[c1] = a W^(c1-4),  Y_g[c2] = P_g V^(c2-4),
+//   W = e(psi df), V = e(8 psi df), P_g = e(psi f_g(36)),  e(t) = exp(2 pi i t),
+// and one row's model over two groups is a complex product summed over
+// components, M[c1][g, c2] = sum_k X_k[c1] Y_kg[c2]: a 16 x 16 real tile
+// of v_mfma_f32_16x16x4f32 with rows (Re c1, Im c1), columns (2 groups x
+// c2) and K = 2 components x (Re, Im of Y):
+//   A[c1][2q] = Xr,  A[c1][2q+1] = -Xi,  A[8+c1][2q] = Xi,  A[8+c1][2q+1] = Xr,
+//   B[2q][n] = Yr,  B[2q+1][n] = Yi   ->   D[c1][n] = Re XY,  D[8+c1][n] = Im XY.
+// The phases psi df, 8 psi df and psi f_g(36) are formed and reduced to
+// [-1/2, 1/2] turns in FP64 (as k_dft_split forms its phases), their sines
+// and cosines in FP32, and the powers W^1..4, V^1..4 by at most two
+// products (W^-k = conj W^k).  Every IF must have the same channel step
+// (the host checks it).  Sums run in FP32 over a tile of 32 components and
+// are folded into FP64 after every tile.
+//
+// A block is 4 waves, each with its own row and working on its own: lane
+// j reads component j mod 32 of a tile straight from the table, a tile
+// ahead.  For a tile, lane j of a wave forms, for component
+// j mod 32, half of X (c1 = 4 (j / 32) ..) and Y of two groups (lanes
+// 0..31 groups 0, 1; lanes 32..63 groups 2, 3) into the wave's operand
+// area, then the wave runs 16 matrix steps per output tile, each lane
+// reading its own A and B elements, the second half's before the first
+// half's steps.
+constexpr int kMfTile = 32;              // components a tile
+// floats a component: X = (Xr, Xi, -Xi) x 8 and Y = 4 groups x 8 x (Yr, Yi),
+// each padded by 4 so the 16-byte writes of 8 consecutive lanes (and the
+// reads) fall on distinct banks
+constexpr int kMfX = 28;
+constexpr int kMfY = 68;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// e(t) for a phase in turns formed in FP64: reduced to [-1/2, 1/2] first
+__device__ __forceinline__ float2 eturn(double t) {
+  float sn, cs;
+  sincos_turns_poly((float)(t - rint(t)), &sn, &cs);
+  return make_float2(cs, sn);
+}
+
+// z^(k - 4) for k = 0..7 from z, by at most two products
+__device__ __forceinline__ void powers_m4(float2 z, float2 (&o)[8]) {
+  const float2 z2 = cmul(z, z), z3 = cmul(z2, z), z4 = cmul(z2, z2);
+  o[0] = make_float2(z4.x, -z4.y);
+  o[1] = make_float2(z3.x, -z3.y);
+  o[2] = make_float2(z2.x, -z2.y);
+  o[3] = make_float2(z.x, -z.y);
+  o[4] = make_float2(1.0f, 0.0f);
+  o[5] = z;
+  o[6] = z2;
+  o[7] = z3;
+}
+
+// ngrp groups of 64 channels: group g = IF g / nchunk64, channels
+// 64 (g mod nchunk64) ..; d.fdel the same for every IF.
+__global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __shared__ float s_op[4][kMfTile * (kMfX + kMfY)];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  float *sX = s_op[wv];
+  float *sY = sX + kMfTile * kMfX;
+  // matrix-step roles: A row i, K index kk; B column n = i
+  const int i = lane & 15, kk = lane >> 4;
+  const int c1A = i & 7, riA = i >> 3, partA = kk & 1;
+  const int selA = riA == 0 ? (partA ? 2 : 0) : (partA ? 0 : 1);
+  const int offA = (kk >> 1) * kMfX + c1A * 3 + selA;            // + 2q kMfX
+  const int offB = (kk >> 1) * kMfY + (i >> 3) * 16 + (i & 7) * 2 + partA;  // + 2q kMfY + 32 t
+  const double df = d.fdel[0];
+  const long long nquad = (d.nrow + 3) / 4;
+  const int ntg = (ngrp + 3) / 4;  // tile groups of up to 4 channel groups
+  for (long long rq = blockIdx.x; rq < nquad; rq += gridDim.x) {
+    const long long row = rq * 4 + wv;
+    const bool valid = row < d.nrow;
+    const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
+    const double u = valid ? rp[d.ilocu] : 0.0f, v = valid ? rp[d.ilocv] : 0.0f, w = valid ? rp[d.ilocw] : 0.0f;
+    for (int tg = 0; tg < ntg; ++tg) {
+      // this lane's two groups of the tile group and their centre frequencies
+      const int gl0 = 4 * tg + 2 * (lane >> 5);
+      double fc[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int g = min(gl0 + h, ngrp - 1);
+        const int isel = g / nchunk64, ch = (g - isel * nchunk64) * 64 + 36;
+        fc[h] = d.fbase[isel] + (double)ch * d.fdel[isel];
+      }
+      const bool have[2] = {gl0 < ngrp, gl0 + 1 < ngrp};
+      f4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+      double o0[4] = {0.0, 0.0, 0.0, 0.0}, o1[4] = {0.0, 0.0, 0.0, 0.0};
+      // this lane's component of the next tile, loaded a tile ahead
+      const int kc = lane & 31, h = lane >> 5;
+      double nx = 0.0, ny = 0.0, nz = 0.0;
+      float na = 0.0f;
+      auto fetch = [&](int k0) {
+        const int k = k0 + kc;
+        const bool ok = k < d.ncomp;
+        nx = ok ? d.cx[k] : 0.0;
+        ny = ok ? d.cy[k] : 0.0;
+        nz = ok ? d.cz[k] : 0.0;
+        na = ok ? d.camp[k] : 0.0f;
+      };
+      fetch(0);
+      for (int k0 = 0; k0 < d.ncomp; k0 += kMfTile) {
+        const double cxk = nx, cyk = ny, czk = nz;
+        const float a = na;
+        fetch(k0 + kMfTile);
+        {
+          // lane = (component kc, half h): X of c1 = 4h .. 4h + 3, Y of this
+          // half's two groups
+          const double psi = fma(w, czk, fma(v, cyk, u * cxk));
+          float2 vp[8];
+          powers_m4(eturn(psi * (8.0 * df)), vp);
+          // W^(c1 - 4) for this half's c1 (selects, not an indexed array)
+          const float2 z1 = eturn(psi * df), z2 = cmul(z1, z1), z3 = cmul(z2, z1), z4 = cmul(z2, z2);
+          float2 wc[4];
+          wc[0] = h ? make_float2(1.0f, 0.0f) : make_float2(z4.x, -z4.y);
+          wc[1] = h ? z1 : make_float2(z3.x, -z3.y);
+          wc[2] = h ? z2 : make_float2(z2.x, -z2.y);
+          wc[3] = h ? z3 : make_float2(z1.x, -z1.y);
+          float *x = sX + kc * kMfX + 12 * h;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float xr = a * wc[c].x, xi = a * wc[c].y;
+            x[3 * c] = xr; x[3 * c + 1] = xi; x[3 * c + 2] = -xi;
+          }
+          float *y = sY + kc * kMfY + 32 * h;
+#pragma unroll
+          for (int hh = 0; hh < 2; ++hh) {
+            const float2 pg = have[hh] ? eturn(psi * fc[hh]) : make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+              const float2 z = cmul(pg, vp[c]);
+              y[16 * hh + 2 * c] = z.x;
+              y[16 * hh + 2 * c + 1] = z.y;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // operands: the second half's loads are issued before the first
+        // half's matrix steps (scheduling barriers keep the order)
+        float av[kMfTile / 2], bv0[kMfTile / 2], bv1[kMfTile / 2];
+#pragma unroll
+        for (int q = 0; q < kMfTile / 4; ++q) {
+          av[q] = sX[offA + 2 * q * kMfX];
+          bv0[q] = sY[offB + 2 * q * kMfY];
+          bv1[q] = sY[offB + 2 * q * kMfY + 32];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = kMfTile / 4; q < kMfTile / 2; ++q) {
+          av[q] = sX[offA + 2 * q * kMfX];
+          bv0[q] = sY[offB + 2 * q * kMfY];
+          bv1[q] = sY[offB + 2 * q * kMfY + 32];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < kMfTile / 4; ++q) {
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv0[q], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv1[q], acc1, 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = kMfTile / 4; q < kMfTile / 2; ++q) {
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv0[q], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv1[q], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          o0[r] += (double)acc0[r];
+          o1[r] += (double)acc1[r];
+        }
+        acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        acc1 = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      // lane l < 32 holds Re of rows 4 (l / 16) + r, lane l + 32 the Im of
+      // the same c1; column n = l mod 16 is (group 2t + n / 8, c2 = n mod 8)
+      double im0[4], im1[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        im0[r] = __shfl_xor(o0[r], 32);
+        im1[r] = __shfl_xor(o1[r], 32);
+      }
+      if (valid && lane < 32) {
+        const int n = lane & 15, c2 = n & 7;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int g = 4 * tg + 2 * t + (n >> 3);
+          if (g >= ngrp) continue;
+          const int isel = g / nchunk64, cbase = (g - isel * nchunk64) * 64;
+          float *p = d.vis + row * (long long)d.lrec + d.nrparm + (d.IFb + isel) * d.incif +
+                     (d.chanb + cbase) * d.incf + d.stokb * d.incs;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int c1 = 4 * (lane >> 4) + r;
+            const int c = c1 + 8 * c2;
+            const double re = t ? o1[r] : o0[r], imv = t ? im1[r] : im0[r];
+            apply_op(d, p + c * d.incf, (float)re, (float)imv);
+          }
+        }
+      }
+    }
+  }
+}
+
 // The sums of the nsplit component parts of every (row, IF, channel),
 // added in part order in FP64, applied to the row as k_dft_split applies
 // its own.
@@ -745,11 +960,52 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
   return kOK;
 }
 
+// Whether k_dft_mfma can run this descriptor: point components, whole
+// 64-channel groups and one channel step for every IF (within 1e-12; the
+// kernel uses the first IF's).  make_desc has left the per-IF lines in
+// st->hfreq.
+bool mfma_ok(const State *st, const Desc &d) {
+  if (model_gauss_off(st->type) >= 0 || model_is_spec(st->type)) return false;
+  if (d.nchsel % 64 != 0 || !st->hfreq) return false;
+  const double d0 = st->hfreq[d.nIFsel];
+  for (int i = 1; i < d.nIFsel; ++i)
+    if (std::fabs(st->hfreq[d.nIFsel + i] - d0) > 1e-12 * std::fabs(d0)) return false;
+  return true;
+}
+
+// Resident workgroups per CU of a kernel (cached).
+int occupancy_of(State *st, const void *fn) {
+  auto it = st->occ.find(fn);
+  if (it != st->occ.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kRows, 0) != hipSuccess || nb < 1) nb = 1;
+  st->occ[fn] = nb;
+  return nb;
+}
+
 // wgPerCU > 0: a persistent grid of wgPerCU workgroups per CU striding over
 // the items; wgPerCU = 0: one workgroup per item, placed by the dispatcher
-// as CUs free up.
+// as CUs free up.  cpt = 64 selects k_dft_mfma where mfma_ok allows it (a
+// persistent grid of the resident workgroups over row quads), else 32.
 int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipStream_t stream = nullptr) {
   if (d.nitems <= 0) return kOK;
+  if (cpt == 64) {
+    if (mfma_ok(st, d)) {
+      const long long nquad = (d.nrow + 3) / 4;
+      const long long grid =
+          std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)k_dft_mfma));
+      const int nchunk64 = d.nchsel / 64;
+      hipLaunchKernelGGL(k_dft_mfma, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
+                         stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64);
+      OBIT_HIP_TRY(hipGetLastError());
+      return kOK;
+    }
+    // the VALU kernel at 32 channels a thread, its items counted for that
+    Desc d32 = d;
+    d32.nchunk = (d.nchsel + 31) / 32;
+    d32.nitems = (long long)d.nrowblk * d.nIFsel * d32.nchunk * std::max(d.nsplit, 1);
+    return launch_split(st, d32, 32, tile, wgPerCU, stream);
+  }
   const bool g = model_gauss_off(st->type) >= 0, s = model_is_spec(st->type);
   KernelFn fn = pick_split(cpt, tile, g, s, st->prec);
   long long grid = wgPerCU > 0 ? (long long)st->nCU * wgPerCU : d.nitems;
@@ -795,16 +1051,7 @@ int run_device(State *st, const GPUInfo *gi, float *vis, long long nrow, bool fp
 
 // ---- the per-buffer launch plan (ObitCUDASkyModelDFTCalc) ----
 
-// Resident workgroups per CU of a kernel (cached).
-int occupancy(State *st, KernelFn fn) {
-  const void *key = (const void *)fn;
-  auto it = st->occ.find(key);
-  if (it != st->occ.end()) return it->second;
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, key, kRows, 0) != hipSuccess || nb < 1) nb = 1;
-  st->occ[key] = nb;
-  return nb;
-}
+int occupancy(State *st, KernelFn fn) { return occupancy_of(st, (const void *)fn); }
 
 struct BufPlan {
   int cpt = 32, tile = 64, nsplit = 1, kpart = 1;
@@ -1245,7 +1492,9 @@ int ObitHIPSkyModelDFTCalcNoAlloc(GPUInfo *gpuInfo, const float *d_model, int nc
   d.cx = cx; d.cy = cy; d.cz = cz;
   d.camp = camp; d.cg = cg;
   st->lastPath = 0;
-  return launch_split(st, d, st->cpt, st->tile, st->wgPerCU);
+  // the VALU kernel (the matrix-core kernel's eligibility reads the host
+  // copy of the lines, which this call does not make)
+  return launch_split(st, d, st->cpt == 64 ? 32 : st->cpt, st->tile, st->wgPerCU);
 }
 
 int ObitHIPSkyModelDFTSetSpecRef(GPUInfo *gpuInfo, const double *refFreq, int n) {
@@ -1340,7 +1589,8 @@ int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo) {
 int ObitHIPSkyModelDFTSetVariant(GPUInfo *gpuInfo, int cpt, int tile, int wgPerCU) {
   State *st = find_state(gpuInfo);
   if (!st) return obit::fail(obit::kState, "DFTSetVariant before DFTInit");
-  if (cpt != 8 && cpt != 16 && cpt != 32) return obit::fail(obit::kInvalid, "cpt must be 8, 16 or 32");
+  if (cpt != 8 && cpt != 16 && cpt != 32 && cpt != 64)
+    return obit::fail(obit::kInvalid, "cpt must be 8, 16, 32 or 64 (the matrix-core kernel)");
   if (tile != 32 && tile != 64 && tile != 128) return obit::fail(obit::kInvalid, "tile must be 32, 64 or 128");
   if (wgPerCU < 1 || wgPerCU > 8) return obit::fail(obit::kInvalid, "wgPerCU must be 1..8");
   st->cpt = cpt; st->tile = tile; st->wgPerCU = wgPerCU;
@@ -1413,6 +1663,24 @@ int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps)
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (ms < best) { best = ms; bc = cpts[ci]; bt = tiles[ti]; bw = wgs[wi]; }
+      }
+    }
+  }
+  if (!rc) {
+    // the matrix-core kernel, where the model and channels allow it
+    obit::dft::Desc d, dw;
+    rc = obit::dft::make_desc(st, gpuInfo, tmp, n, 64, d);
+    if (!rc) rc = obit::dft::make_desc(st, gpuInfo, tmp, std::min(n, (long long)obit::dft::kRows), 64, dw);
+    if (!rc && obit::dft::mfma_ok(st, d)) {
+      rc = obit::dft::launch_split(st, dw, 64, bt, bw);
+      if (!rc) {
+        (void)hipEventRecord(e0, st->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = obit::dft::launch_split(st, d, 64, bt, bw);
+        (void)hipEventRecord(e1, st->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; bc = 64; }
       }
     }
   }

@@ -45,7 +45,7 @@ def w1():
     return uv, sky
 
 
-@pytest.mark.parametrize("variant", [(16, 32, 1), (16, 64, 2), (32, 64, 2), (32, 128, 4)])
+@pytest.mark.parametrize("variant", [(16, 32, 1), (16, 64, 2), (32, 64, 2), (32, 128, 4), (64, 64, 2)])
 def test_point_model_vs_fp64(oracle, w1, variant):
     uv, sky = w1
     out = _run(uv, sky, op="repl", variant=variant)
@@ -401,3 +401,44 @@ def test_buffer_component_parts(oracle, kind):
             rel = np.abs(da[..., :2] - db[..., :2]) / (np.abs(db[..., :2]).max() + 1e-30)
             assert rel[m].max() <= 1e-4
             assert np.allclose(da[..., 2][m], db[..., 2][m], rtol=1e-4)
+
+
+@pytest.mark.parametrize("op", ["repl", "sub", "div"])
+def test_matrix_core_kernel(oracle, op):
+    """The matrix-core kernel (variant cpt 64: 64-channel groups as a
+    complex product on v_mfma_f32_16x16x4f32) on workload 2's 4 IF x 64
+    channel layout: the model against the FP64 oracle, and subtract /
+    divide against the VALU kernel; a ragged last row quad."""
+    uv, sky, _ = synth.workload(2, rows=1027, ncomp=700)
+    out = _run(uv, sky, op=op, variant=(64, 64, 2))
+    if op == "repl":
+        ref = oracle.dft_model_fp64(uv, sky)
+        err = np.abs(_model_from(uv, out) - ref).max() / np.abs(ref).max()
+        print(f"matrix-core kernel: max {err:.3e}", flush=True)
+        assert err <= TOL
+        return
+    ref_run = _run(uv, sky, op=op, variant=(32, 64, 2))
+    peak = np.abs(oracle.dft_model_fp64(uv, sky)).max()
+    da = out[:, uv.nrparm:].reshape(uv.nrow, -1, 3)
+    db = ref_run[:, uv.nrparm:].reshape(uv.nrow, -1, 3)
+    if op == "sub":
+        assert np.abs(da[..., :2] - db[..., :2]).max() <= 2 * TOL * peak
+        assert np.array_equal(da[..., 2], db[..., 2])
+    else:
+        # weight x |model|: the kernels' models agree within 2 TOL of the peak
+        w0 = uv.buffer[:, uv.nrparm:].reshape(uv.nrow, -1, 3)[..., 2]
+        assert np.all(np.abs(da[..., 2] - db[..., 2]) <= 2 * TOL * peak * np.abs(w0) + 1e-30)
+
+
+def test_matrix_core_kernel_falls_back(oracle):
+    """A channel selection that is not whole 64-channel groups (48 of 64)
+    and a Gaussian model: the cpt 64 variant runs the VALU kernel, with the
+    same result as cpt 32."""
+    uv, sky, _ = synth.workload(2, rows=300, ncomp=300)
+    a = _run(uv, sky, op="repl", variant=(64, 64, 2), startChannel=9, numberChannel=48)
+    b = _run(uv, sky, op="repl", variant=(32, 64, 2), startChannel=9, numberChannel=48)
+    assert np.array_equal(a, b)
+    uv3, sky3, _ = synth.workload(3, rows=64)
+    a = _run(uv3, sky3, op="repl", variant=(64, 64, 2))
+    b = _run(uv3, sky3, op="repl", variant=(32, 64, 2))
+    assert np.array_equal(a, b)
