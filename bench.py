@@ -94,6 +94,8 @@ def bench_dft(lib, steps, warmup, sample_rows, variant=None, accuracy=True):
         with SkyModelDFT(sub, sky, op="repl") as s2:
             s2.set_vis(sub.buffer)
             if kind == "split":
+                # the benched kernel variant
+                s2.set_variant(*(res["variant"][k] for k in ("cpt", "tile", "wg_per_cu")))
                 s2.calc_resident()
             else:
                 st2 = s2.stream()

@@ -98,16 +98,20 @@ int ObitHIPSkyModelDFTCalcFP64(GPUInfo *gpuInfo);
 /* CalcResident captured once into a hipGraph and replayed. */
 int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo);
 /* Kernel shape: channels per thread (16 or 32), component tile (32, 64
- * or 128), workgroups per CU of the persistent grid (1..8).  Autotune
- * times every shape on a copy of the first resident rows and keeps the
- * fastest: sampleRows rows, raised where needed so that the widest grid
- * gets at least 8 rounds of work items, and capped at the resident rows. */
+ * or 128), workgroups per CU of the persistent grid (1..8); cpt 64 selects
+ * the matrix-core kernel (point models whose selected channels are whole
+ * groups of 64 with one channel step for every IF; other launches run the
+ * 32-channel VALU kernel), which ignores tile and wgPerCU.  Autotune times
+ * every shape on a copy of the first resident rows and keeps the fastest:
+ * sampleRows rows, raised where needed so that the widest grid gets at
+ * least 8 rounds of work items, and capped at the resident rows. */
 int ObitHIPSkyModelDFTSetVariant(GPUInfo *gpuInfo, int cpt, int tile, int wgPerCU);
 int ObitHIPSkyModelDFTGetVariant(GPUInfo *gpuInfo, int *cpt, int *tile, int *wgPerCU);
 int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps);
 /* Sin/cos policy: 0 hardware sin/cos for the centre phasor and the
  * per-channel step; 1 hardware for the centre phasor, Taylor polynomial
- * for the step (default); 2 polynomial for both. */
+ * for the step (default); 2 polynomial for both.  The matrix-core kernel
+ * reads its centre phasors P and its steps W, V the same way. */
 int ObitHIPSkyModelDFTSetPrecision(GPUInfo *gpuInfo, int prec);
 /* hipStream_t on which the library launches for this gpuInfo. */
 void *ObitHIPSkyModelDFTStream(GPUInfo *gpuInfo);

@@ -282,8 +282,8 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
 //   B[2q][n] = Yr,  B[2q+1][n] = Yi   ->   D[c1][n] = Re XY,  D[8+c1][n] = Im XY.
 // The phases psi df, 8 psi df and psi f_g(36) are formed and reduced to
 // [-1/2, 1/2] turns in FP64 (as k_dft_split forms its phases), their sines
-// and cosines in FP32, and the powers W^1..4, V^1..4 by at most two
-// products (W^-k = conj W^k).  Every IF must have the same channel step
+// and cosines in FP32 (hardware or polynomial, by the precision setting),
+// and the powers W^1..4, V^1..4 by at most two products (W^-k = conj W^k).  Every IF must have the same channel step
 // (the host checks it).  Sums run in FP32 over a tile of 32 components and
 // are folded into FP64 after every tile.
 //
@@ -306,10 +306,13 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// e(t) for a phase in turns formed in FP64: reduced to [-1/2, 1/2] first
+// e(t) for a phase in turns formed in FP64: reduced to [-1/2, 1/2] first,
+// then the hardware sin / cos (HW) or the Taylor polynomial
+template <bool HW>
 __device__ __forceinline__ float2 eturn(double t) {
   float sn, cs;
-  sincos_turns_poly((float)(t - rint(t)), &sn, &cs);
+  if (HW) sincos_turns_hw((float)(t - rint(t)), &sn, &cs);
+  else sincos_turns_poly((float)(t - rint(t)), &sn, &cs);
   return make_float2(cs, sn);
 }
 
@@ -328,6 +331,10 @@ __device__ __forceinline__ void powers_m4(float2 z, float2 (&o)[8]) {
 
 // ngrp groups of 64 channels: group g = IF g / nchunk64, channels
 // 64 (g mod nchunk64) ..; d.fdel the same for every IF.
+// PREC as k_dft_split's: 0 hardware sin / cos for every phasor, 1 hardware
+// for the centre phasors P and the polynomial for the steps W and V, 2 the
+// polynomial for all.
+template <int PREC>
 __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   __shared__ float s_op[4][kMfTile * (kMfX + kMfY)];
@@ -384,9 +391,9 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
           // half's two groups
           const double psi = fma(w, czk, fma(v, cyk, u * cxk));
           float2 vp[8];
-          powers_m4(eturn(psi * (8.0 * df)), vp);
+          powers_m4(eturn<PREC == 0>(psi * (8.0 * df)), vp);
           // W^(c1 - 4) for this half's c1 (selects, not an indexed array)
-          const float2 z1 = eturn(psi * df), z2 = cmul(z1, z1), z3 = cmul(z2, z1), z4 = cmul(z2, z2);
+          const float2 z1 = eturn<PREC == 0>(psi * df), z2 = cmul(z1, z1), z3 = cmul(z2, z1), z4 = cmul(z2, z2);
           float2 wc[4];
           wc[0] = h ? make_float2(1.0f, 0.0f) : make_float2(z4.x, -z4.y);
           wc[1] = h ? z1 : make_float2(z3.x, -z3.y);
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
           float *y = sY + kc * kMfY + 32 * h;
 #pragma unroll
           for (int hh = 0; hh < 2; ++hh) {
-            const float2 pg = have[hh] ? eturn(psi * fc[hh]) : make_float2(0.0f, 0.0f);
+            const float2 pg = have[hh] ? eturn<PREC != 2>(psi * fc[hh]) : make_float2(0.0f, 0.0f);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
               const float2 z = cmul(pg, vp[c]);
@@ -992,10 +999,11 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
   if (cpt == 64) {
     if (mfma_ok(st, d)) {
       const long long nquad = (d.nrow + 3) / 4;
+      void (*fn)(Desc, int, int) = st->prec == 0 ? k_dft_mfma<0> : st->prec == 2 ? k_dft_mfma<2> : k_dft_mfma<1>;
       const long long grid =
-          std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)k_dft_mfma));
+          std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
       const int nchunk64 = d.nchsel / 64;
-      hipLaunchKernelGGL(k_dft_mfma, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
+      hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
                          stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64);
       OBIT_HIP_TRY(hipGetLastError());
       return kOK;
