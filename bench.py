@@ -859,6 +859,7 @@ def main():
         dft_extra = {}
 
     achieved = FLOP_PER_EVAL * dft["evals_per_step"] / (dft["kernel_ms"] * 1e-3) / 1e12
+    mfma = dft["variant"]["cpt"] == 64
     line = {
         "metric": "vis x comp DFT evals/s (split precision, max|dV| vs FP64 stated), 1 MI355X",
         "value": dft["evals_per_s"],
@@ -876,19 +877,22 @@ def main():
             "workload": "64 antennas (2016 baselines) x 450 integrations, 4 IF x 64 ch, RR/LL, "
                         "10,000 power-law points within 0.7 deg; DFT subtract",
             "rows": dft["rows"], "channels": dft["channels"], "components": dft["components"],
-            "precision": "FP64 phase + range reduction, FP32 sincos/recurrence/sums, FP64 tile folds",
+            "precision": ("FP64 phases + range reduction, FP32 sincos and phasor powers, FP32 matrix-core "
+                          "sums over 32-component tiles folded into FP64" if mfma else
+                          "FP64 phase + range reduction, FP32 sincos/recurrence/sums, FP64 tile folds"),
+            "kernel": "k_dft_mfma (v_mfma_f32_16x16x4f32)" if mfma else "k_dft_split (VALU)",
             "variant": dft["variant"],
             **dft_extra,
             **(pcie or {}),
             "kernel_ms": dft["kernel_ms"],
         },
         "roofline": {
-            "bound": "valu",
+            "bound": "mfma" if mfma else "valu",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": read_traffic("dft_traffic.json"),
+            "traffic": read_traffic("dft_mfma_traffic.json" if mfma else "dft_traffic.json"),
         },
         "cpu_baseline": cpu,
         "gridding": grid,
