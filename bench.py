@@ -877,10 +877,11 @@ def main():
             "workload": "64 antennas (2016 baselines) x 450 integrations, 4 IF x 64 ch, RR/LL, "
                         "10,000 power-law points within 0.7 deg; DFT subtract",
             "rows": dft["rows"], "channels": dft["channels"], "components": dft["components"],
-            "precision": ("FP64 phases + range reduction, FP32 sincos and phasor powers, FP32 matrix-core "
-                          "sums over 32-component tiles folded into FP64" if mfma else
+            "precision": ("FP64 phases + range reduction, FP32 sincos and phasor powers, operands split "
+                          "into FP16 hi + lo parts (hi*hi + hi*lo + lo*hi on the matrix cores), FP32 sums "
+                          "over 32-component tiles folded into FP64" if mfma else
                           "FP64 phase + range reduction, FP32 sincos/recurrence/sums, FP64 tile folds"),
-            "kernel": "k_dft_mfma (v_mfma_f32_16x16x4f32)" if mfma else "k_dft_split (VALU)",
+            "kernel": "k_dft_mfma (v_mfma_f32_16x16x32_f16, FP16 hi/lo parts)" if mfma else "k_dft_split (VALU)",
             "variant": dft["variant"],
             **dft_extra,
             **(pcie or {}),

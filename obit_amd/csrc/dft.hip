@@ -276,31 +276,42 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
 //   W = e(psi df), V = e(8 psi df), P_g = e(psi f_g(36)),  e(t) = exp(2 pi i t),
 // and one row's model over two groups is a complex product summed over
 // components, M[c1][g, c2] = sum_k X_k[c1] Y_kg[c2]: a 16 x 16 real tile
-// of v_mfma_f32_16x16x4f32 with rows (Re c1, Im c1), columns (2 groups x
-// c2) and K = 2 components x (Re, Im of Y):
-//   A[c1][2q] = Xr,  A[c1][2q+1] = -Xi,  A[8+c1][2q] = Xi,  A[8+c1][2q+1] = Xr,
-//   B[2q][n] = Yr,  B[2q+1][n] = Yi   ->   D[c1][n] = Re XY,  D[8+c1][n] = Im XY.
+// with rows (Re c1, Im c1), columns (2 groups x c2) and the sum over
+// (component, Re / Im of Y):
+//   A[c1][(k,0)] = Xr,  A[c1][(k,1)] = -Xi,  A[8+c1][(k,0)] = Xi,  A[8+c1][(k,1)] = Xr,
+//   B[(k,0)][n] = Yr,  B[(k,1)][n] = Yi   ->   D[c1][n] = Re XY,  D[8+c1][n] = Im XY.
+// It runs on v_mfma_f32_16x16x32_f16 (16 components a step) with every
+// operand split into FP16 parts, x = x_h + x_l (x_h = fp16(x), x_l =
+// fp16(x - x_h)), and the three products A_h B_h + A_h B_l + A_l B_h summed
+// in FP32: the dropped A_l B_l and the rounding of the parts are below
+// 2^-21 of a term.  X is scaled by 2^ashift (the table's largest
+// amplitude to [2^13, 2^14), set when the model is loaded), so that the
+// parts stay in the FP16 range; the scale is taken
+// out exactly when the sums are written.
 // The phases psi df, 8 psi df and psi f_g(36) are formed and reduced to
 // [-1/2, 1/2] turns in FP64 (as k_dft_split forms its phases), their sines
 // and cosines in FP32 (hardware or polynomial, by the precision setting),
-// and the powers W^1..4, V^1..4 by at most two products (W^-k = conj W^k).  Every IF must have the same channel step
-// (the host checks it).  Sums run in FP32 over a tile of 32 components and
-// are folded into FP64 after every tile.
+// and the powers W^1..4, V^1..4 by at most two products (W^-k = conj W^k).
+// Every IF must have the same channel step (the host checks it).
 //
 // A block is 4 waves, each with its own row and working on its own: lane
 // j reads component j mod 32 of a tile straight from the table, a tile
-// ahead.  For a tile, lane j of a wave forms, for component
-// j mod 32, half of X (c1 = 4 (j / 32) ..) and Y of two groups (lanes
-// 0..31 groups 0, 1; lanes 32..63 groups 2, 3) into the wave's operand
-// area, then the wave runs 16 matrix steps per output tile, each lane
-// reading its own A and B elements, the second half's before the first
-// half's steps.
-constexpr int kMfTile = 32;              // components a tile
-// floats a component: X = (Xr, Xi, -Xi) x 8 and Y = 4 groups x 8 x (Yr, Yi),
-// each padded by 4 so the 16-byte writes of 8 consecutive lanes (and the
-// reads) fall on distinct banks
-constexpr int kMfX = 28;
-constexpr int kMfY = 68;
+// ahead.  For a tile, lane j of a wave forms, for component j mod 32, half
+// of X (c1 = 4 (j / 32) ..) and Y of two groups (lanes 0..31 groups 0, 1;
+// lanes 32..63 groups 2, 3), splits them and writes the parts to the
+// wave's operand area; then the wave runs 2 steps x 2 output tiles x 3
+// products, each lane reading its 8 FP16 values of A and B as one 16-byte
+// load.
+constexpr int kMfTile = 32;  // components a tile
+// Operand area of a wave, in 16-byte slots (4 components x 2 FP16 values
+// of one row): slot (b, r) of component block b = k / 4 and row r is
+// b * R + r, with R = 17 (A: 16 rows + 1) or 33 (B: 2 tiles x 16 columns +
+// 1); the odd block stride keeps the 4-byte writes of 32 consecutive
+// components and the 16-byte reads of 16 consecutive rows on distinct
+// banks.
+constexpr int kMfRA = 17, kMfRB = 33;
+constexpr int kMfSlotsA = 8 * kMfRA, kMfSlotsB = 8 * kMfRB;
+constexpr int kMfSlots = 2 * kMfSlotsA + 2 * kMfSlotsB;  // A_h, A_l, B_h, B_l
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -329,25 +340,40 @@ __device__ __forceinline__ void powers_m4(float2 z, float2 (&o)[8]) {
   o[7] = z3;
 }
 
+// FP16 parts of (x, y), each pair packed in a 32-bit word (x in the low
+// half): hi = fp16(x), fp16(y) rounded to nearest; lo = fp16(x - hi.x),
+// fp16(y - hi.y), the differences formed in FP32 from the FP16 values.
+// Three instructions (the compiler's own sequence re-forms x - hi in
+// FP32 with two conversions and two subtractions).
+__device__ __forceinline__ void split_pair(float x, float y, unsigned &hi, unsigned &lo) {
+  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
+      "v_fma_mixlo_f16 %1, -%0, 1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %1, -%0, 1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(hi), "=&v"(lo)
+      : "v"(x), "v"(y));
+}
+
 // ngrp groups of 64 channels: group g = IF g / nchunk64, channels
 // 64 (g mod nchunk64) ..; d.fdel the same for every IF.
 // PREC as k_dft_split's: 0 hardware sin / cos for every phasor, 1 hardware
 // for the centre phasors P and the polynomial for the steps W and V, 2 the
 // polynomial for all.
 template <int PREC>
-__global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64) {
+__global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64, int ashift) {
   typedef float f4 __attribute__((ext_vector_type(4)));
-  __shared__ float s_op[4][kMfTile * (kMfX + kMfY)];
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  __shared__ uint4 s_op[4][kMfSlots];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  float *sX = s_op[wv];
-  float *sY = sX + kMfTile * kMfX;
-  // matrix-step roles: A row i, K index kk; B column n = i
-  const int i = lane & 15, kk = lane >> 4;
-  const int c1A = i & 7, riA = i >> 3, partA = kk & 1;
-  const int selA = riA == 0 ? (partA ? 2 : 0) : (partA ? 0 : 1);
-  const int offA = (kk >> 1) * kMfX + c1A * 3 + selA;            // + 2q kMfX
-  const int offB = (kk >> 1) * kMfY + (i >> 3) * 16 + (i & 7) * 2 + partA;  // + 2q kMfY + 32 t
+  uint4 *sAh = s_op[wv], *sAl = sAh + kMfSlotsA, *sBh = sAl + kMfSlotsA, *sBl = sBh + kMfSlotsB;
+  // matrix-step roles: A row / B column i, component block qd of a step
+  const int i = lane & 15, qd = lane >> 4;
+  const int rdA = qd * kMfRA + i, rdB = qd * kMfRB + i;
+  // writer roles: component kc (word kc & 3 of block kc / 4), half h
+  const int kc = lane & 31, h = lane >> 5;
+  unsigned *wA = reinterpret_cast<unsigned *>(sAh + (kc >> 2) * kMfRA + 4 * h) + (kc & 3);
+  unsigned *wB = reinterpret_cast<unsigned *>(sBh + (kc >> 2) * kMfRB + 16 * h) + (kc & 3);
+  constexpr int kLoA = 4 * kMfSlotsA, kLoB = 4 * kMfSlotsB;  // words from the _h to the _l array
   const double df = d.fdel[0];
   const long long nquad = (d.nrow + 3) / 4;
   const int ntg = (ngrp + 3) / 4;  // tile groups of up to 4 channel groups
@@ -357,38 +383,38 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
     const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
     const double u = valid ? rp[d.ilocu] : 0.0f, v = valid ? rp[d.ilocv] : 0.0f, w = valid ? rp[d.ilocw] : 0.0f;
     for (int tg = 0; tg < ntg; ++tg) {
-      // this lane's two groups of the tile group and their centre frequencies
-      const int gl0 = 4 * tg + 2 * (lane >> 5);
+      // this lane's two groups of the tile group and their centre
+      // frequencies (a group past the last is worked on as the last one:
+      // its columns are not written)
+      const int gl0 = 4 * tg + 2 * h;
       double fc[2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int g = min(gl0 + h, ngrp - 1);
+      for (int hh = 0; hh < 2; ++hh) {
+        const int g = min(gl0 + hh, ngrp - 1);
         const int isel = g / nchunk64, ch = (g - isel * nchunk64) * 64 + 36;
-        fc[h] = d.fbase[isel] + (double)ch * d.fdel[isel];
+        fc[hh] = d.fbase[isel] + (double)ch * d.fdel[isel];
       }
-      const bool have[2] = {gl0 < ngrp, gl0 + 1 < ngrp};
-      f4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
       double o0[4] = {0.0, 0.0, 0.0, 0.0}, o1[4] = {0.0, 0.0, 0.0, 0.0};
-      // this lane's component of the next tile, loaded a tile ahead
-      const int kc = lane & 31, h = lane >> 5;
+      // this lane's component of the next tile, loaded a tile ahead (index
+      // clamped into the table, zero amplitude past its end)
       double nx = 0.0, ny = 0.0, nz = 0.0;
       float na = 0.0f;
       auto fetch = [&](int k0) {
         const int k = k0 + kc;
-        const bool ok = k < d.ncomp;
-        nx = ok ? d.cx[k] : 0.0;
-        ny = ok ? d.cy[k] : 0.0;
-        nz = ok ? d.cz[k] : 0.0;
-        na = ok ? d.camp[k] : 0.0f;
+        const int kk = min(k, d.ncomp - 1);
+        nx = d.cx[kk];
+        ny = d.cy[kk];
+        nz = d.cz[kk];
+        const float am = d.camp[kk];
+        na = k < d.ncomp ? am : 0.0f;
       };
-      fetch(0);
+      if (d.ncomp > 0) fetch(0);
       for (int k0 = 0; k0 < d.ncomp; k0 += kMfTile) {
         const double cxk = nx, cyk = ny, czk = nz;
         const float a = na;
         fetch(k0 + kMfTile);
+        const float as = ldexpf(a, ashift);
         {
-          // lane = (component kc, half h): X of c1 = 4h .. 4h + 3, Y of this
-          // half's two groups
           const double psi = fma(w, czk, fma(v, cyk, u * cxk));
           float2 vp[8];
           powers_m4(eturn<PREC == 0>(psi * (8.0 * df)), vp);
@@ -399,65 +425,56 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
           wc[1] = h ? z1 : make_float2(z3.x, -z3.y);
           wc[2] = h ? z2 : make_float2(z2.x, -z2.y);
           wc[3] = h ? z3 : make_float2(z1.x, -z1.y);
-          float *x = sX + kc * kMfX + 12 * h;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            const float xr = a * wc[c].x, xi = a * wc[c].y;
-            x[3 * c] = xr; x[3 * c + 1] = xi; x[3 * c + 2] = -xi;
+            unsigned xh, xl;
+            split_pair(as * wc[c].x, as * wc[c].y, xh, xl);
+            // row c1 = 4h + c: (Xr, -Xi); row 8 + c1: (Xi, Xr)
+            wA[4 * c] = xh ^ 0x80000000u;
+            wA[4 * c + kLoA] = xl ^ 0x80000000u;
+            wA[4 * (c + 8)] = __builtin_amdgcn_alignbit(xh, xh, 16);
+            wA[4 * (c + 8) + kLoA] = __builtin_amdgcn_alignbit(xl, xl, 16);
           }
-          float *y = sY + kc * kMfY + 32 * h;
 #pragma unroll
           for (int hh = 0; hh < 2; ++hh) {
-            const float2 pg = have[hh] ? eturn<PREC != 2>(psi * fc[hh]) : make_float2(0.0f, 0.0f);
+            const float2 pg = eturn<PREC != 2>(psi * fc[hh]);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
               const float2 z = cmul(pg, vp[c]);
-              y[16 * hh + 2 * c] = z.x;
-              y[16 * hh + 2 * c + 1] = z.y;
+              unsigned yh, yl;
+              split_pair(z.x, z.y, yh, yl);
+              wB[4 * (8 * hh + c)] = yh;
+              wB[4 * (8 * hh + c) + kLoB] = yl;
             }
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // operands: the second half's loads are issued before the first
-        // half's matrix steps (scheduling barriers keep the order)
-        float av[kMfTile / 2], bv0[kMfTile / 2], bv1[kMfTile / 2];
+        f4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int q = 0; q < kMfTile / 4; ++q) {
-          av[q] = sX[offA + 2 * q * kMfX];
-          bv0[q] = sY[offB + 2 * q * kMfY];
-          bv1[q] = sY[offB + 2 * q * kMfY + 32];
+        for (int s = 0; s < 2; ++s) {
+          const h8 ah = __builtin_bit_cast(h8, sAh[rdA + 4 * s * kMfRA]);
+          const h8 al = __builtin_bit_cast(h8, sAl[rdA + 4 * s * kMfRA]);
+          const h8 b0h = __builtin_bit_cast(h8, sBh[rdB + 4 * s * kMfRB]);
+          const h8 b0l = __builtin_bit_cast(h8, sBl[rdB + 4 * s * kMfRB]);
+          const h8 b1h = __builtin_bit_cast(h8, sBh[rdB + 4 * s * kMfRB + 16]);
+          const h8 b1l = __builtin_bit_cast(h8, sBl[rdB + 4 * s * kMfRB + 16]);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b0h, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h, acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b0l, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l, acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b0h, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h, acc1, 0, 0, 0);
         }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int q = kMfTile / 4; q < kMfTile / 2; ++q) {
-          av[q] = sX[offA + 2 * q * kMfX];
-          bv0[q] = sY[offB + 2 * q * kMfY];
-          bv1[q] = sY[offB + 2 * q * kMfY + 32];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int q = 0; q < kMfTile / 4; ++q) {
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv0[q], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv1[q], acc1, 0, 0, 0);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int q = kMfTile / 4; q < kMfTile / 2; ++q) {
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv0[q], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv1[q], acc1, 0, 0, 0);
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           o0[r] += (double)acc0[r];
           o1[r] += (double)acc1[r];
         }
-        acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        acc1 = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
       // lane l < 32 holds Re of rows 4 (l / 16) + r, lane l + 32 the Im of
       // the same c1; column n = l mod 16 is (group 2t + n / 8, c2 = n mod 8)
@@ -468,6 +485,7 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
         im1[r] = __shfl_xor(o1[r], 32);
       }
       if (valid && lane < 32) {
+        const double unscale = ldexp(1.0, -ashift);
         const int n = lane & 15, c2 = n & 7;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -481,7 +499,7 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
             const int c1 = 4 * (lane >> 4) + r;
             const int c = c1 + 8 * c2;
             const double re = t ? o1[r] : o0[r], imv = t ? im1[r] : im0[r];
-            apply_op(d, p + c * d.incf, (float)re, (float)imv);
+            apply_op(d, p + c * d.incf, (float)(re * unscale), (float)(imv * unscale));
           }
         }
       }
@@ -594,6 +612,20 @@ __global__ void k_prep_comps(const float *model, int size, int n, int gaussOff, 
   }
 }
 
+// Largest |amplitude| of n components into out[0] (one block).
+__global__ __launch_bounds__(256) void k_amp_max(const float *camp, int n, float *out) {
+  __shared__ float s[256];
+  float m = 0.0f;
+  for (int k = threadIdx.x; k < n; k += 256) m = fmaxf(m, fabsf(camp[k]));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
 // Spectral amplitude per (component, selected IF, selected channel) in
 // FP64, stored FP32.  Parametrised (type Spec): a*exp(sum_t c_t L^(t+1)),
 // L = ln(nu/nu_ref), coefficients at coefOff (src/ObitSkyModel.c:2528-2552,
@@ -673,6 +705,8 @@ struct State {
   double *cx = nullptr, *cy = nullptr, *cz = nullptr;
   float *camp = nullptr, *cg = nullptr;
   int capComp = 0;
+  int ampShift = 0;          // k_dft_mfma's amplitude scale exponent
+  float *dAmax = nullptr;    // largest |amplitude| of the table (device)
   const float *modelDev = nullptr;  // caller's table, for the spectral prep
   // spectral table
   float *tab = nullptr;
@@ -999,12 +1033,12 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
   if (cpt == 64) {
     if (mfma_ok(st, d)) {
       const long long nquad = (d.nrow + 3) / 4;
-      void (*fn)(Desc, int, int) = st->prec == 0 ? k_dft_mfma<0> : st->prec == 2 ? k_dft_mfma<2> : k_dft_mfma<1>;
+      void (*fn)(Desc, int, int, int) = st->prec == 0 ? k_dft_mfma<0> : st->prec == 2 ? k_dft_mfma<2> : k_dft_mfma<1>;
       const long long grid =
           std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
       const int nchunk64 = d.nchsel / 64;
       hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
-                         stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64);
+                         stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64, st->ampShift);
       OBIT_HIP_TRY(hipGetLastError());
       return kOK;
     }
@@ -1279,6 +1313,21 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
                        mi->model[0], size, n, gOff, st->cx, st->cy, st->cz, st->camp, st->cg);
     OBIT_HIP_CHECK(hipGetLastError());
   }
+  // k_dft_mfma's scale: the largest amplitude to [2^13, 2^14)
+  st->ampShift = 0;
+  if (n > 0) {
+    if (!st->dAmax) OBIT_HIP_CHECK(hipMalloc(&st->dAmax, sizeof(float)));
+    hipLaunchKernelGGL(obit::dft::k_amp_max, dim3(1), dim3(256), 0, st->stream, st->camp, n, st->dAmax);
+    OBIT_HIP_CHECK(hipGetLastError());
+    float amax = 0.0f;
+    OBIT_HIP_CHECK(hipMemcpyAsync(&amax, st->dAmax, sizeof(float), hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+    int e = 0;
+    if (amax > 0.0f && std::isfinite(amax)) {
+      (void)std::frexp(amax, &e);
+      st->ampShift = 14 - e;
+    }
+  }
   OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
   st->modelReady = true;
   st->tabReady = false;
@@ -1409,6 +1458,7 @@ void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
   obit::dft::destroy_graph(st);
   (void)hipFree(st->cx); (void)hipFree(st->cy); (void)hipFree(st->cz); (void)hipFree(st->camp); (void)hipFree(st->cg);
   (void)hipFree(st->tab); (void)hipFree(st->dfreq); (void)hipFree(st->stage); (void)hipFree(st->part);
+  (void)hipFree(st->dAmax);
   for (hipEvent_t e : st->events) (void)hipEventDestroy(e);
   if (st->hfreq) (void)hipHostFree(st->hfreq);
   if (st->ownVis) (void)hipFree(st->dvis);

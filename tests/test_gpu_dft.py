@@ -406,9 +406,10 @@ def test_buffer_component_parts(oracle, kind):
 @pytest.mark.parametrize("op", ["repl", "sub", "div"])
 def test_matrix_core_kernel(oracle, op):
     """The matrix-core kernel (variant cpt 64: 64-channel groups as a
-    complex product on v_mfma_f32_16x16x4f32) on workload 2's 4 IF x 64
-    channel layout: the model against the FP64 oracle, and subtract /
-    divide against the VALU kernel; a ragged last row quad."""
+    complex product on v_mfma_f32_16x16x32_f16 over FP16 operand parts) on
+    workload 2's 4 IF x 64 channel layout: the model against the FP64
+    oracle, and subtract / divide against the VALU kernel; a ragged last
+    row quad."""
     uv, sky, _ = synth.workload(2, rows=1027, ncomp=700)
     out = _run(uv, sky, op=op, variant=(64, 64, 2))
     if op == "repl":
@@ -428,6 +429,38 @@ def test_matrix_core_kernel(oracle, op):
         # weight x |model|: the kernels' models agree within 2 TOL of the peak
         w0 = uv.buffer[:, uv.nrparm:].reshape(uv.nrow, -1, 3)[..., 2]
         assert np.all(np.abs(da[..., 2] - db[..., 2]) <= 2 * TOL * peak * np.abs(w0) + 1e-30)
+
+
+def test_matrix_core_amplitude_scale(oracle):
+    """The matrix-core kernel scales its FP16 operand parts by the table's
+    largest amplitude, taken again at every model load: through one
+    SkyModelDFT (replace), a table of tiny amplitudes, then large ones
+    with a 1e6 dynamic range, against the FP64 oracle; then an all-zero
+    table, whose model is zero."""
+    import dataclasses
+    from obit_amd.skymodel import SkyModelDFT
+    uv, sky, _ = synth.workload(2, rows=258, ncomp=500)
+    tiny = sky.comps.copy()
+    tiny[:, 0] *= 1e-7
+    wide = sky.comps.copy()
+    wide[:, 0] *= 3e3
+    wide[::7, 0] *= 1e-6
+    zero = sky.comps.copy()
+    zero[:, 0] = 0.0
+    with SkyModelDFT(uv, dataclasses.replace(sky, comps=tiny), op="repl") as sm:
+        sm.set_variant(64, 64, 2)
+        for name, comps in (("tiny", tiny), ("wide", wide), ("zero", zero)):
+            sm.set_model(comps)
+            sm.set_vis(uv.buffer)
+            sm.calc_resident()
+            got = _model_from(uv, sm.get_vis())
+            if name == "zero":
+                assert np.all(got == 0)
+                continue
+            ref = oracle.dft_model_fp64(uv, dataclasses.replace(sky, comps=comps))
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"{name}: max {err:.3e}", flush=True)
+            assert err <= TOL
 
 
 def test_matrix_core_kernel_falls_back(oracle):
