@@ -299,9 +299,13 @@ __global__ __launch_bounds__(kRows) void k_dft_split(Desc d) {
 // ahead.  For a tile, lane j of a wave forms, for component j mod 32, half
 // of X (c1 = 4 (j / 32) ..) and Y of two groups (lanes 0..31 groups 0, 1;
 // lanes 32..63 groups 2, 3), splits them and writes the parts to the
-// wave's operand area; then the wave runs 2 steps x 2 output tiles x 3
-// products, each lane reading its 8 FP16 values of A and B as one 16-byte
-// load.
+// wave's operand area.  The first half-wave forms W and its powers, the
+// second V and its, and one lane exchange gives each lane both.  The wave
+// then runs 2 steps x 2 output tiles x 3 products, each lane reading its 8
+// FP16 values of A and B as one 16-byte load; the steps are interleaved
+// with the next tile's set-up, and the FP32 sums are folded into FP64
+// every 4 tiles.  3 waves per SIMD (168 registers) with 3 blocks per CU
+// (LDS).
 constexpr int kMfTile = 32;  // components a tile
 // Operand area of a wave, in 16-byte slots (4 components x 2 FP16 values
 // of one row): slot (b, r) of component block b = k / 4 and row r is
@@ -340,6 +344,17 @@ __device__ __forceinline__ void powers_m4(float2 z, float2 (&o)[8]) {
   o[7] = z3;
 }
 
+// a value of the first half-wave (lanes 0..31) and of the second
+// (32..63) in every lane: lo[l] = x[l mod 32], hi[l] = x[32 + l mod 32]
+// (v_permlane32_swap exchanges lanes 32..63 of its first operand with
+// lanes 0..31 of its second)
+__device__ __forceinline__ void swap_halves(float2 x, float2 &lo, float2 &hi) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(x.x), __float_as_uint(x.x), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(x.y), __float_as_uint(x.y), false, false);
+  lo = make_float2(__uint_as_float(a[0]), __uint_as_float(b[0]));
+  hi = make_float2(__uint_as_float(a[1]), __uint_as_float(b[1]));
+}
+
 // FP16 parts of (x, y), each pair packed in a 32-bit word (x in the low
 // half): hi = fp16(x), fp16(y) rounded to nearest; lo = fp16(x - hi.x),
 // fp16(y - hi.y), the differences formed in FP32 from the FP16 values.
@@ -359,7 +374,7 @@ __device__ __forceinline__ void split_pair(float x, float y, unsigned &hi, unsig
 // for the centre phasors P and the polynomial for the steps W and V, 2 the
 // polynomial for all.
 template <int PREC>
-__global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64, int ashift) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_dft_mfma(Desc d, int ngrp, int nchunk64, int ashift) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   typedef _Float16 h8 __attribute__((ext_vector_type(8)));
   __shared__ uint4 s_op[4][kMfSlots];
@@ -408,18 +423,31 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
         const float am = d.camp[kk];
         na = k < d.ncomp ? am : 0.0f;
       };
-      if (d.ncomp > 0) fetch(0);
-      for (int k0 = 0; k0 < d.ncomp; k0 += kMfTile) {
-        const double cxk = nx, cyk = ny, czk = nz;
-        const float a = na;
-        fetch(k0 + kMfTile);
+      // a tile's set-up (the X and Y parts of its components into the
+      // operand area)
+      auto setup = [&](double cxk, double cyk, double czk, float a) {
         const float as = ldexpf(a, ashift);
         {
           const double psi = fma(w, czk, fma(v, cyk, u * cxk));
+          // the first half forms W = e(psi df) and its powers, the second
+          // V = e(8 psi df) and its; one lane exchange gives every lane both
+          const float2 s1 = eturn<PREC == 0>(psi * (h ? 8.0 * df : df)), s2 = cmul(s1, s1), s3 = cmul(s2, s1),
+                       s4 = cmul(s2, s2);
+          float2 z1, z2, z3, z4, q1, q2, q3, q4;
+          swap_halves(s1, z1, q1);
+          swap_halves(s2, z2, q2);
+          swap_halves(s3, z3, q3);
+          swap_halves(s4, z4, q4);
           float2 vp[8];
-          powers_m4(eturn<PREC == 0>(psi * (8.0 * df)), vp);
+          vp[0] = make_float2(q4.x, -q4.y);
+          vp[1] = make_float2(q3.x, -q3.y);
+          vp[2] = make_float2(q2.x, -q2.y);
+          vp[3] = make_float2(q1.x, -q1.y);
+          vp[4] = make_float2(1.0f, 0.0f);
+          vp[5] = q1;
+          vp[6] = q2;
+          vp[7] = q3;
           // W^(c1 - 4) for this half's c1 (selects, not an indexed array)
-          const float2 z1 = eturn<PREC == 0>(psi * df), z2 = cmul(z1, z1), z3 = cmul(z2, z1), z4 = cmul(z2, z2);
           float2 wc[4];
           wc[0] = h ? make_float2(1.0f, 0.0f) : make_float2(z4.x, -z4.y);
           wc[1] = h ? z1 : make_float2(z3.x, -z3.y);
@@ -448,28 +476,56 @@ __global__ __launch_bounds__(256) void k_dft_mfma(Desc d, int ngrp, int nchunk64
             }
           }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      };
+      if (d.ncomp > 0) {
+        fetch(0);
+        const double cxk = nx, cyk = ny, czk = nz;
+        const float a = na;
+        fetch(kMfTile);
+        setup(cxk, cyk, czk, a);
+      }
+      // 4 tiles between folds into FP64 (tiles past the table have zero
+      // amplitude); a tile's matrix steps run interleaved with the next
+      // tile's set-up, whose writes follow the reads of this tile's
+      // operands (a wave's LDS accesses complete in order)
+      for (int k4 = 0; k4 < d.ncomp; k4 += 4 * kMfTile) {
         f4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const h8 ah = __builtin_bit_cast(h8, sAh[rdA + 4 * s * kMfRA]);
-          const h8 al = __builtin_bit_cast(h8, sAl[rdA + 4 * s * kMfRA]);
-          const h8 b0h = __builtin_bit_cast(h8, sBh[rdB + 4 * s * kMfRB]);
-          const h8 b0l = __builtin_bit_cast(h8, sBl[rdB + 4 * s * kMfRB]);
-          const h8 b1h = __builtin_bit_cast(h8, sBh[rdB + 4 * s * kMfRB + 16]);
-          const h8 b1l = __builtin_bit_cast(h8, sBl[rdB + 4 * s * kMfRB + 16]);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b0h, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h, acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b0l, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l, acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b0h, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h, acc1, 0, 0, 0);
+        for (int kt = 0; kt < 4; ++kt) {
+          const int k0 = k4 + kt * kMfTile;
+          asm volatile("" ::: "memory");
+          h8 ah[2], al[2], b0h[2], b0l[2], b1h[2], b1l[2];
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            ah[s] = __builtin_bit_cast(h8, sAh[rdA + 4 * s * kMfRA]);
+            al[s] = __builtin_bit_cast(h8, sAl[rdA + 4 * s * kMfRA]);
+            b0h[s] = __builtin_bit_cast(h8, sBh[rdB + 4 * s * kMfRB]);
+            b0l[s] = __builtin_bit_cast(h8, sBl[rdB + 4 * s * kMfRB]);
+            b1h[s] = __builtin_bit_cast(h8, sBh[rdB + 4 * s * kMfRB + 16]);
+            b1l[s] = __builtin_bit_cast(h8, sBl[rdB + 4 * s * kMfRB + 16]);
+          }
+          asm volatile("" ::: "memory");
+          {
+            const double cxk = nx, cyk = ny, czk = nz;
+            const float a = na;
+            fetch(k0 + 2 * kMfTile);
+            setup(cxk, cyk, czk, a);
+          }
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[s], b0h[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[s], b1h[s], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[s], b0l[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[s], b1l[s], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[s], b0h[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[s], b1h[s], acc1, 0, 0, 0);
+          }
+#pragma unroll
+          for (int g = 0; g < 12; ++g) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 18, 0);
+          }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           o0[r] += (double)acc0[r];
