@@ -392,7 +392,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const double df = d.fdel[0];
   const long long nquad = (d.nrow + 3) / 4;
   const int ntg = (ngrp + 3) / 4;  // tile groups of up to 4 channel groups
-  for (long long rq = blockIdx.x; rq < nquad; rq += gridDim.x) {
+  // items: (component part, row quad); with d.part set, a part's FP64 sums
+  // go to the part scratch for k_dft_combine
+  const int nsplit = max(d.nsplit, 1);
+  for (long long it = blockIdx.x; it < nquad * nsplit; it += gridDim.x) {
+    const int part = (int)(it / nquad);
+    const long long rq = it - (long long)part * nquad;
+    const int kBeg = part * d.kpart, kEnd = nsplit > 1 ? min(d.ncomp, kBeg + d.kpart) : d.ncomp;
     const long long row = rq * 4 + wv;
     const bool valid = row < d.nrow;
     const float *rp = d.vis + (valid ? row : 0LL) * (long long)d.lrec;
@@ -416,12 +422,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       float na = 0.0f;
       auto fetch = [&](int k0) {
         const int k = k0 + kc;
-        const int kk = min(k, d.ncomp - 1);
+        const int kk = min(k, kEnd - 1);
         nx = d.cx[kk];
         ny = d.cy[kk];
         nz = d.cz[kk];
         const float am = d.camp[kk];
-        na = k < d.ncomp ? am : 0.0f;
+        na = k < kEnd ? am : 0.0f;
       };
       // a tile's set-up (the X and Y parts of its components into the
       // operand area)
@@ -477,18 +483,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
           }
         }
       };
-      if (d.ncomp > 0) {
-        fetch(0);
+      if (kEnd > kBeg) {
+        fetch(kBeg);
         const double cxk = nx, cyk = ny, czk = nz;
         const float a = na;
-        fetch(kMfTile);
+        fetch(kBeg + kMfTile);
         setup(cxk, cyk, czk, a);
       }
       // 4 tiles between folds into FP64 (tiles past the table have zero
       // amplitude); a tile's matrix steps run interleaved with the next
       // tile's set-up, whose writes follow the reads of this tile's
       // operands (a wave's LDS accesses complete in order)
-      for (int k4 = 0; k4 < d.ncomp; k4 += 4 * kMfTile) {
+      for (int k4 = kBeg; k4 < kEnd; k4 += 4 * kMfTile) {
         f4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
@@ -550,12 +556,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
           const int isel = g / nchunk64, cbase = (g - isel * nchunk64) * 64;
           float *p = d.vis + row * (long long)d.lrec + d.nrparm + (d.IFb + isel) * d.incif +
                      (d.chanb + cbase) * d.incf + d.stokb * d.incs;
+          double2 *pp = d.part ? d.part + (((long long)part * d.nrow + row) * d.nIFsel + isel) * d.nchsel + cbase
+                               : nullptr;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int c1 = 4 * (lane >> 4) + r;
             const int c = c1 + 8 * c2;
             const double re = t ? o1[r] : o0[r], imv = t ? im1[r] : im0[r];
-            apply_op(d, p + c * d.incf, (float)(re * unscale), (float)(imv * unscale));
+            if (pp) pp[c] = make_double2(re * unscale, imv * unscale);
+            else apply_op(d, p + c * d.incf, (float)(re * unscale), (float)(imv * unscale));
           }
         }
       }
@@ -1070,6 +1079,11 @@ bool mfma_ok(const State *st, const Desc &d) {
   return true;
 }
 
+typedef void (*MfmaFn)(Desc, int, int, int);
+MfmaFn mfma_fn(const State *st) {
+  return st->prec == 0 ? k_dft_mfma<0> : st->prec == 2 ? k_dft_mfma<2> : k_dft_mfma<1>;
+}
+
 // Resident workgroups per CU of a kernel (cached).
 int occupancy_of(State *st, const void *fn) {
   auto it = st->occ.find(fn);
@@ -1089,9 +1103,9 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
   if (cpt == 64) {
     if (mfma_ok(st, d)) {
       const long long nquad = (d.nrow + 3) / 4;
-      void (*fn)(Desc, int, int, int) = st->prec == 0 ? k_dft_mfma<0> : st->prec == 2 ? k_dft_mfma<2> : k_dft_mfma<1>;
+      MfmaFn fn = mfma_fn(st);
       const long long grid =
-          std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
+          std::min<long long>(nquad * std::max(d.nsplit, 1), (long long)st->nCU * occupancy_of(st, (const void *)fn));
       const int nchunk64 = d.nchsel / 64;
       hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
                          stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64, st->ampShift);
@@ -1176,10 +1190,44 @@ struct BufPlan {
 // 4, 8, 8, 8, 8, 4 row blocks ran at 5.7e12 evaluations/s, against 5.5e12
 // with five equal segments, 4.9e12 with three, and 5.3-5.45e12 with 12,
 // 23 or 32 parts.
+//
+// Where k_dft_mfma can run the buffer (mfma_ok) it does, with its own
+// items: (component part, row quad).  The component list is cut into parts
+// of whole 128-component groups (the kernel's fold interval) so that the
+// buffer makes about 20 rounds of its resident workgroups, and segments
+// are sized as above with the middle ones a sixth of the buffer.
 BufPlan plan_buffer(State *st, const Desc &d, size_t rowBytes) {
   (void)rowBytes;
   BufPlan p;
   p.kpart = std::max(d.ncomp, 1);
+  if (mfma_ok(st, d)) {
+    p.cpt = 64;
+    const long long slots = (long long)std::max(st->nCU, 1) * occupancy_of(st, (const void *)mfma_fn(st));
+    const long long nquad = (d.nrow + 3) / 4;
+    if (d.ncomp > 0) {
+      const double chans = (double)d.nrow * d.nIFsel * d.nchsel;
+      const int want = (int)std::min<long long>(16, std::max<long long>(1, (20 * slots + nquad - 1) / nquad));
+      const int kpt = ((d.ncomp + want - 1) / want + 127) / 128 * 128;
+      const int ns = (d.ncomp + kpt - 1) / kpt;
+      if (ns > 1 && (double)ns * chans * sizeof(double2) <= double(size_t(1) << 30)) {
+        p.nsplit = ns;
+        p.kpart = kpt;
+      }
+    }
+    const long long perBlock = (long long)(kRows / 4) * p.nsplit;  // items of one row block
+    const int round = (int)std::max<long long>(1, slots / perBlock);
+    const int mid = std::max(2 * round, (d.nrowblk + 5) / 6);
+    int b = 0;
+    for (int i = 0; b < d.nrowblk; ++i) {
+      int want = (i == 0) ? round : mid;
+      const int left = d.nrowblk - b;
+      if (left <= want) want = left;
+      else if (left < want + round) want = left - round;
+      p.segBlk.push_back(want);
+      b += want;
+    }
+    return p;
+  }
   const int n16 = (d.nchsel + 15) / 16, n32 = (d.nchsel + 31) / 32;
   p.cpt = (n16 * 16 / 0.9 < n32 * 32.0) ? 16 : 32;
   const int nchunk = p.cpt == 16 ? n16 : n32;
