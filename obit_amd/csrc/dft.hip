@@ -771,6 +771,9 @@ struct State {
   float *camp = nullptr, *cg = nullptr;
   int capComp = 0;
   int ampShift = 0;          // k_dft_mfma's amplitude scale exponent
+  int npoint = 0;            // Gaussian tables: leading components with no Gaussian terms
+  float *cgG = nullptr;      // ... and the Gaussian terms of the rest, [3][ncomp - npoint]
+  int cgGCap = 0;
   float *dAmax = nullptr;    // largest |amplitude| of the table (device)
   const float *modelDev = nullptr;  // caller's table, for the spectral prep
   // spectral table
@@ -1070,13 +1073,24 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
 // 64-channel groups and one channel step for every IF (within 1e-12; the
 // kernel uses the first IF's).  make_desc has left the per-IF lines in
 // st->hfreq.
-bool mfma_ok(const State *st, const Desc &d) {
-  if (model_gauss_off(st->type) >= 0 || model_is_spec(st->type)) return false;
+bool channels_ok(const State *st, const Desc &d) {
   if (d.nchsel % 64 != 0 || !st->hfreq) return false;
   const double d0 = st->hfreq[d.nIFsel];
   for (int i = 1; i < d.nIFsel; ++i)
     if (std::fabs(st->hfreq[d.nIFsel + i] - d0) > 1e-12 * std::fabs(d0)) return false;
   return true;
+}
+
+bool mfma_ok(const State *st, const Desc &d) {
+  return model_gauss_off(st->type) < 0 && !model_is_spec(st->type) && channels_ok(st, d);
+}
+
+// A Gaussian table (no spectra) whose leading npoint components have no
+// Gaussian terms (SetMod orders it so), subtracted over whole rows: the
+// points run on k_dft_mfma and the Gaussians on the VALU kernel, each
+// launch subtracting its own part of the model.
+bool mixed_ok(const State *st, const Desc &d) {
+  return st->type == GPUModTypeGauss && st->npoint > 0 && d.op == kSub && !d.part && channels_ok(st, d);
 }
 
 typedef void (*MfmaFn)(Desc, int, int, int);
@@ -1111,6 +1125,28 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
                          stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64, st->ampShift);
       OBIT_HIP_TRY(hipGetLastError());
       return kOK;
+    }
+    if (mixed_ok(st, d)) {
+      const long long nquad = (d.nrow + 3) / 4;
+      MfmaFn fn = mfma_fn(st);
+      const long long grid = std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
+      const int nchunk64 = d.nchsel / 64;
+      Desc dp = d;
+      dp.ncomp = st->npoint;
+      hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
+                         stream ? stream : st->stream, dp, d.nIFsel * nchunk64, nchunk64, st->ampShift);
+      OBIT_HIP_TRY(hipGetLastError());
+      if (st->npoint == d.ncomp) return kOK;
+      Desc dg = d;
+      dg.cx = d.cx + st->npoint;
+      dg.cy = d.cy + st->npoint;
+      dg.cz = d.cz + st->npoint;
+      dg.camp = d.camp + st->npoint;
+      dg.cg = st->cgG;
+      dg.ncomp = d.ncomp - st->npoint;
+      dg.nchunk = (d.nchsel + 31) / 32;
+      dg.nitems = (long long)d.nrowblk * d.nIFsel * dg.nchunk;
+      return launch_split(st, dg, 32, tile, wgPerCU, stream);
     }
     // the VALU kernel at 32 channels a thread, its items counted for that
     Desc d32 = d;
@@ -1421,6 +1457,55 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
                        mi->model[0], size, n, gOff, st->cx, st->cy, st->cz, st->camp, st->cg);
     OBIT_HIP_CHECK(hipGetLastError());
   }
+  // a Gaussian table: its components without Gaussian terms first (every
+  // array in the same order), and the Gaussian terms of the rest in cgG
+  st->npoint = 0;
+  if (n > 0 && st->type == GPUModTypeGauss) {
+    OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+    std::vector<double> x(n), y(n), z(n);
+    std::vector<float> a(n), g(3 * (size_t)n);
+    OBIT_HIP_CHECK(hipMemcpyAsync(x.data(), st->cx, n * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipMemcpyAsync(y.data(), st->cy, n * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipMemcpyAsync(z.data(), st->cz, n * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipMemcpyAsync(a.data(), st->camp, n * sizeof(float), hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipMemcpyAsync(g.data(), st->cg, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st->stream));
+    OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+    std::vector<int> order;
+    order.reserve(n);
+    for (int k = 0; k < n; ++k)
+      if (g[k] == 0.0f && g[n + k] == 0.0f && g[2 * (size_t)n + k] == 0.0f) order.push_back(k);
+    const int np = (int)order.size();
+    for (int k = 0; k < n; ++k)
+      if (!(g[k] == 0.0f && g[n + k] == 0.0f && g[2 * (size_t)n + k] == 0.0f)) order.push_back(k);
+    if (np > 0) {
+      const int ng = n - np;
+      std::vector<double> x2(n), y2(n), z2(n);
+      std::vector<float> a2(n), g2(3 * (size_t)n), gG(3 * (size_t)std::max(ng, 1));
+      for (int i = 0; i < n; ++i) {
+        const int k = order[i];
+        x2[i] = x[k]; y2[i] = y[k]; z2[i] = z[k]; a2[i] = a[k];
+        for (int t = 0; t < 3; ++t) g2[t * (size_t)n + i] = g[t * (size_t)n + k];
+        if (i >= np)
+          for (int t = 0; t < 3; ++t) gG[t * (size_t)ng + (i - np)] = g[t * (size_t)n + k];
+      }
+      OBIT_HIP_CHECK(hipMemcpyAsync(st->cx, x2.data(), n * sizeof(double), hipMemcpyHostToDevice, st->stream));
+      OBIT_HIP_CHECK(hipMemcpyAsync(st->cy, y2.data(), n * sizeof(double), hipMemcpyHostToDevice, st->stream));
+      OBIT_HIP_CHECK(hipMemcpyAsync(st->cz, z2.data(), n * sizeof(double), hipMemcpyHostToDevice, st->stream));
+      OBIT_HIP_CHECK(hipMemcpyAsync(st->camp, a2.data(), n * sizeof(float), hipMemcpyHostToDevice, st->stream));
+      OBIT_HIP_CHECK(hipMemcpyAsync(st->cg, g2.data(), 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, st->stream));
+      if (ng > st->cgGCap) {
+        (void)hipFree(st->cgG);
+        st->cgG = nullptr;
+        OBIT_HIP_CHECK(hipMalloc(&st->cgG, 3 * (size_t)ng * sizeof(float)));
+        st->cgGCap = ng;
+      }
+      if (ng > 0)
+        OBIT_HIP_CHECK(hipMemcpyAsync(st->cgG, gG.data(), 3 * (size_t)ng * sizeof(float), hipMemcpyHostToDevice, st->stream));
+      // the copies read these host vectors: done before they go
+      OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+      st->npoint = np;
+    }
+  }
   // k_dft_mfma's scale: the largest amplitude to [2^13, 2^14)
   st->ampShift = 0;
   if (n > 0) {
@@ -1567,6 +1652,7 @@ void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
   (void)hipFree(st->cx); (void)hipFree(st->cy); (void)hipFree(st->cz); (void)hipFree(st->camp); (void)hipFree(st->cg);
   (void)hipFree(st->tab); (void)hipFree(st->dfreq); (void)hipFree(st->stage); (void)hipFree(st->part);
   (void)hipFree(st->dAmax);
+  (void)hipFree(st->cgG);
   for (hipEvent_t e : st->events) (void)hipEventDestroy(e);
   if (st->hfreq) (void)hipHostFree(st->hfreq);
   if (st->ownVis) (void)hipFree(st->dvis);
@@ -1837,7 +1923,7 @@ int ObitHIPSkyModelDFTAutotune(GPUInfo *gpuInfo, long long sampleRows, int reps)
     obit::dft::Desc d, dw;
     rc = obit::dft::make_desc(st, gpuInfo, tmp, n, 64, d);
     if (!rc) rc = obit::dft::make_desc(st, gpuInfo, tmp, std::min(n, (long long)obit::dft::kRows), 64, dw);
-    if (!rc && obit::dft::mfma_ok(st, d)) {
+    if (!rc && (obit::dft::mfma_ok(st, d) || obit::dft::mixed_ok(st, d))) {
       rc = obit::dft::launch_split(st, dw, 64, bt, bw);
       if (!rc) {
         (void)hipEventRecord(e0, st->stream);

@@ -463,6 +463,38 @@ def test_matrix_core_amplitude_scale(oracle):
             assert err <= TOL
 
 
+def test_mixed_table_subtract(oracle):
+    """A Gaussian-type table with point components (workload 3's 40
+    facets, 20 % Gaussians), subtracted with variant cpt 64: its points run
+    on the matrix-unit kernel and its Gaussians on the VALU kernel.  Against
+    the FP64 oracle and the single VALU launch (cpt 32); then the same
+    table with every Gaussian term zeroed (all points, one matrix-unit
+    launch)."""
+    import dataclasses
+    uv, sky, _ = synth.workload(3, rows=300)
+    assert (sky.comps[:, 4:7] != 0).any(axis=1).sum() > 0
+    for name, s in (("mixed", sky),
+                    ("points", dataclasses.replace(sky, comps=np.concatenate(
+                        [sky.comps[:, :4], np.zeros_like(sky.comps[:, 4:7])], axis=1)))):
+        ref = oracle.dft_model_fp64(uv, s)
+        peak = np.abs(ref).max()
+        a = _run(uv, s, op="sub", variant=(64, 64, 2))
+        b = _run(uv, s, op="sub", variant=(32, 64, 2))
+        d0 = uv.buffer[:, uv.nrparm:].reshape(uv.nrow, uv.nif, uv.nchan, uv.nstok, 3)
+        da = a[:, uv.nrparm:].reshape(d0.shape)
+        db = b[:, uv.nrparm:].reshape(d0.shape)
+        w = d0[..., 0, 2] > 0
+        got = (d0[..., 0, 0] - da[..., 0, 0]) + 1j * (d0[..., 0, 1] - da[..., 0, 1])
+        err = np.abs(got - ref)[w].max()
+        print(f"{name}: max {err / peak:.3e} of peak vs FP64", flush=True)
+        assert err <= TOL * peak + 4e-7 * np.abs(d0[..., 0, :2]).max()
+        assert np.abs(da[..., :2] - db[..., :2]).max() <= 2 * TOL * peak
+        assert np.array_equal(da[..., 2], db[..., 2])
+        # another kernel ran (a fallback to the VALU kernel gives cpt 32's
+        # result bit for bit)
+        assert not np.array_equal(da[..., :2], db[..., :2])
+
+
 def test_matrix_core_kernel_falls_back(oracle):
     """A channel selection that is not whole 64-channel groups (48 of 64)
     and a Gaussian model: the cpt 64 variant runs the VALU kernel, with the
