@@ -771,9 +771,14 @@ struct State {
   float *camp = nullptr, *cg = nullptr;
   int capComp = 0;
   int ampShift = 0;          // k_dft_mfma's amplitude scale exponent
-  int npoint = 0;            // Gaussian tables: leading components with no Gaussian terms
-  float *cgG = nullptr;      // ... and the Gaussian terms of the rest, [3][ncomp - npoint]
-  int cgGCap = 0;
+  // Gaussian tables: a copy ordered points first (mx, my, mz, mamp), its
+  // npoint leading components without Gaussian terms, and the Gaussian
+  // terms of the rest ([3][ncomp - npoint]), for the mixed launch
+  int npoint = 0;
+  double *mx = nullptr, *my = nullptr, *mz = nullptr;
+  float *mamp = nullptr, *cgG = nullptr;
+  void *mbuf = nullptr;
+  size_t mbufCap = 0;
   float *dAmax = nullptr;    // largest |amplitude| of the table (device)
   const float *modelDev = nullptr;  // caller's table, for the spectral prep
   // spectral table
@@ -1085,10 +1090,11 @@ bool mfma_ok(const State *st, const Desc &d) {
   return model_gauss_off(st->type) < 0 && !model_is_spec(st->type) && channels_ok(st, d);
 }
 
-// A Gaussian table (no spectra) whose leading npoint components have no
-// Gaussian terms (SetMod orders it so), subtracted over whole rows: the
-// points run on k_dft_mfma and the Gaussians on the VALU kernel, each
-// launch subtracting its own part of the model.
+// A Gaussian table (no spectra) with npoint components without Gaussian
+// terms (SetMod keeps a copy of the table ordered points first),
+// subtracted over whole rows: the points run on k_dft_mfma and the
+// Gaussians on the VALU kernel, each launch subtracting its own part of the
+// model.
 bool mixed_ok(const State *st, const Desc &d) {
   return st->type == GPUModTypeGauss && st->npoint > 0 && d.op == kSub && !d.part && channels_ok(st, d);
 }
@@ -1132,16 +1138,20 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
       const long long grid = std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
       const int nchunk64 = d.nchsel / 64;
       Desc dp = d;
+      dp.cx = st->mx;
+      dp.cy = st->my;
+      dp.cz = st->mz;
+      dp.camp = st->mamp;
       dp.ncomp = st->npoint;
       hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
                          stream ? stream : st->stream, dp, d.nIFsel * nchunk64, nchunk64, st->ampShift);
       OBIT_HIP_TRY(hipGetLastError());
       if (st->npoint == d.ncomp) return kOK;
       Desc dg = d;
-      dg.cx = d.cx + st->npoint;
-      dg.cy = d.cy + st->npoint;
-      dg.cz = d.cz + st->npoint;
-      dg.camp = d.camp + st->npoint;
+      dg.cx = st->mx + st->npoint;
+      dg.cy = st->my + st->npoint;
+      dg.cz = st->mz + st->npoint;
+      dg.camp = st->mamp + st->npoint;
       dg.cg = st->cgG;
       dg.ncomp = d.ncomp - st->npoint;
       dg.nchunk = (d.nchsel + 31) / 32;
@@ -1457,8 +1467,9 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
                        mi->model[0], size, n, gOff, st->cx, st->cy, st->cz, st->camp, st->cg);
     OBIT_HIP_CHECK(hipGetLastError());
   }
-  // a Gaussian table: its components without Gaussian terms first (every
-  // array in the same order), and the Gaussian terms of the rest in cgG
+  // a Gaussian table: a copy with its components without Gaussian terms
+  // first, and the Gaussian terms of the rest (the table itself keeps its
+  // order for every other launch)
   st->npoint = 0;
   if (n > 0 && st->type == GPUModTypeGauss) {
     OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
@@ -1470,39 +1481,42 @@ void ObitCUDASkyModelDFTSetMod(GPUInfo *gpuInfo) {
     OBIT_HIP_CHECK(hipMemcpyAsync(a.data(), st->camp, n * sizeof(float), hipMemcpyDeviceToHost, st->stream));
     OBIT_HIP_CHECK(hipMemcpyAsync(g.data(), st->cg, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st->stream));
     OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+    auto isPoint = [&](int k) { return g[k] == 0.0f && g[n + k] == 0.0f && g[2 * (size_t)n + k] == 0.0f; };
     std::vector<int> order;
     order.reserve(n);
     for (int k = 0; k < n; ++k)
-      if (g[k] == 0.0f && g[n + k] == 0.0f && g[2 * (size_t)n + k] == 0.0f) order.push_back(k);
+      if (isPoint(k)) order.push_back(k);
     const int np = (int)order.size();
     for (int k = 0; k < n; ++k)
-      if (!(g[k] == 0.0f && g[n + k] == 0.0f && g[2 * (size_t)n + k] == 0.0f)) order.push_back(k);
+      if (!isPoint(k)) order.push_back(k);
     if (np > 0) {
       const int ng = n - np;
-      std::vector<double> x2(n), y2(n), z2(n);
-      std::vector<float> a2(n), g2(3 * (size_t)n), gG(3 * (size_t)std::max(ng, 1));
+      // one buffer: mx, my, mz (doubles), mamp, cgG (floats)
+      const size_t need = 3 * (size_t)n * sizeof(double) + (size_t)n * sizeof(float) + 3 * (size_t)ng * sizeof(float);
+      if (need > st->mbufCap) {
+        (void)hipFree(st->mbuf);
+        st->mbuf = nullptr;
+        st->mbufCap = 0;
+        OBIT_HIP_CHECK(hipMalloc(&st->mbuf, need));
+        st->mbufCap = need;
+      }
+      std::vector<unsigned char> h(need);
+      double *hx = reinterpret_cast<double *>(h.data()), *hy = hx + n, *hz = hy + n;
+      float *ha = reinterpret_cast<float *>(hz + n), *hg = ha + n;
       for (int i = 0; i < n; ++i) {
         const int k = order[i];
-        x2[i] = x[k]; y2[i] = y[k]; z2[i] = z[k]; a2[i] = a[k];
-        for (int t = 0; t < 3; ++t) g2[t * (size_t)n + i] = g[t * (size_t)n + k];
+        hx[i] = x[k]; hy[i] = y[k]; hz[i] = z[k]; ha[i] = a[k];
         if (i >= np)
-          for (int t = 0; t < 3; ++t) gG[t * (size_t)ng + (i - np)] = g[t * (size_t)n + k];
+          for (int t = 0; t < 3; ++t) hg[t * (size_t)ng + (i - np)] = g[t * (size_t)n + k];
       }
-      OBIT_HIP_CHECK(hipMemcpyAsync(st->cx, x2.data(), n * sizeof(double), hipMemcpyHostToDevice, st->stream));
-      OBIT_HIP_CHECK(hipMemcpyAsync(st->cy, y2.data(), n * sizeof(double), hipMemcpyHostToDevice, st->stream));
-      OBIT_HIP_CHECK(hipMemcpyAsync(st->cz, z2.data(), n * sizeof(double), hipMemcpyHostToDevice, st->stream));
-      OBIT_HIP_CHECK(hipMemcpyAsync(st->camp, a2.data(), n * sizeof(float), hipMemcpyHostToDevice, st->stream));
-      OBIT_HIP_CHECK(hipMemcpyAsync(st->cg, g2.data(), 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, st->stream));
-      if (ng > st->cgGCap) {
-        (void)hipFree(st->cgG);
-        st->cgG = nullptr;
-        OBIT_HIP_CHECK(hipMalloc(&st->cgG, 3 * (size_t)ng * sizeof(float)));
-        st->cgGCap = ng;
-      }
-      if (ng > 0)
-        OBIT_HIP_CHECK(hipMemcpyAsync(st->cgG, gG.data(), 3 * (size_t)ng * sizeof(float), hipMemcpyHostToDevice, st->stream));
-      // the copies read these host vectors: done before they go
+      OBIT_HIP_CHECK(hipMemcpyAsync(st->mbuf, h.data(), need, hipMemcpyHostToDevice, st->stream));
+      // the copy reads the host vector: done before it goes
       OBIT_HIP_CHECK(hipStreamSynchronize(st->stream));
+      st->mx = reinterpret_cast<double *>(st->mbuf);
+      st->my = st->mx + n;
+      st->mz = st->my + n;
+      st->mamp = reinterpret_cast<float *>(st->mz + n);
+      st->cgG = st->mamp + n;
       st->npoint = np;
     }
   }
@@ -1652,7 +1666,7 @@ void ObitCUDASkyModelDFTShutdown(GPUInfo *gpuInfo) {
   (void)hipFree(st->cx); (void)hipFree(st->cy); (void)hipFree(st->cz); (void)hipFree(st->camp); (void)hipFree(st->cg);
   (void)hipFree(st->tab); (void)hipFree(st->dfreq); (void)hipFree(st->stage); (void)hipFree(st->part);
   (void)hipFree(st->dAmax);
-  (void)hipFree(st->cgG);
+  (void)hipFree(st->mbuf);
   for (hipEvent_t e : st->events) (void)hipEventDestroy(e);
   if (st->hfreq) (void)hipHostFree(st->hfreq);
   if (st->ownVis) (void)hipFree(st->dvis);
