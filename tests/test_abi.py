@@ -46,7 +46,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_reference_entry_points_present():
-    """The ObitCUDA* set Obit's host classes call (SURVEY 8b list)."""
+    """The ObitCUDA* entry points Obit's host classes call
+    (src/ObitGPUSkyModel.c, src/ObitGPUGrid.c, src/ObitGPUBeamModel.c)."""
     from obit_amd import _lib
     exports = _exports(_lib.LIB_PATH)
     for n in ("ObitCUDASkyModelDFTInit", "ObitCUDASkyModelDFTSetMod", "ObitCUDASkyModelDFTCalc",
