@@ -59,7 +59,8 @@ def test_reference_entry_points_present():
               "ObitCUDAUtilAllocHost", "ObitCUDAUtilFreeHost", "ObitCUDAUtilAllocGPU",
               "ObitCUDAUtilFreeGPU", "ObitCUDAUtilHost2GPU", "ObitCUDAUtilHostAny2GPU",
               "ObitCUDAUtilGPU2Host", "ObitCUDAUtilGPUAny2Host", "ObitCUDAUtilMemory",
-              "ObitCUDAUtilPointerType", "CUDAMagicF"):
+              "ObitCUDAUtilPointerType", "CUDAMagicF", "ObitCUDABeamModelDFTInit",
+              "ObitCUDABeamModelDFTSetMod", "ObitCUDABeamModelDFTCalc", "ObitCUDABeamModelDFTShutdown"):
         assert n in exports, n
 
 
