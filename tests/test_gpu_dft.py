@@ -463,6 +463,26 @@ def test_matrix_core_amplitude_scale(oracle):
             assert err <= TOL
 
 
+def test_buffer_matrix_core_single_part(oracle):
+    """A per-buffer call whose table is too short to cut into parts (100
+    point components) runs the matrix-unit kernel on each piece directly:
+    replace against the FP64 oracle, subtract against the resident launch."""
+    from obit_amd.skymodel import SkyModelDFT
+    uv, sky, _ = synth.workload(2, rows=700, ncomp=100)
+    with SkyModelDFT(uv, sky, op="repl", nVisPIO=256) as sm:
+        out = sm.calc(uv.buffer.copy())
+        cpt, nsplit = sm.buffer_shape()
+    assert (cpt, nsplit) == (64, 1)
+    ref = oracle.dft_model_fp64(uv, sky)
+    peak = np.abs(ref).max()
+    err = np.abs(_model_from(uv, out) - ref).max() / peak
+    print(f"single part: max {err:.3e}", flush=True)
+    assert err <= TOL
+    a = _run(uv, sky, op="sub", path="buffer", nVisPIO=256)
+    b = _run(uv, sky, op="sub", path="resident", variant=(64, 64, 2))
+    assert np.array_equal(a, b)
+
+
 def test_mixed_table_subtract(oracle):
     """A Gaussian-type table with point components (workload 3's 40
     facets, 20 % Gaussians), subtracted with variant cpt 64: its points run
