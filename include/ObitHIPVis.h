@@ -99,11 +99,12 @@ int ObitHIPSkyModelDFTCalcFP64(GPUInfo *gpuInfo);
 int ObitHIPSkyModelDFTGraphLaunch(GPUInfo *gpuInfo);
 /* Kernel shape: channels per thread (16 or 32), component tile (32, 64
  * or 128), workgroups per CU of the persistent grid (1..8); cpt 64 selects
- * the matrix-core kernel (point models whose selected channels are whole
- * groups of 64 with one channel step for every IF; for a subtraction with
- * a Gaussian table on such channels, its components without Gaussian
- * terms, the rest on the VALU kernel; other launches run the 32-channel
- * VALU kernel), which ignores tile and wgPerCU.  Autotune times
+ * the matrix-core kernel (point models with one channel step for every
+ * IF, in groups of 64 selected channels, a ragged last group formed in
+ * full and only its selected channels written; for a subtraction with a
+ * Gaussian table on such channels, its components without Gaussian terms,
+ * the rest on the VALU kernel; other launches run the 32-channel VALU
+ * kernel), which ignores tile and wgPerCU.  Autotune times
  * every shape on a copy of the first resident rows and keeps the fastest:
  * sampleRows rows, raised where needed so that the widest grid gets at
  * least 8 rounds of work items, and capped at the resident rows. */

@@ -369,7 +369,8 @@ __device__ __forceinline__ void split_pair(float x, float y, unsigned &hi, unsig
 }
 
 // ngrp groups of 64 channels: group g = IF g / nchunk64, channels
-// 64 (g mod nchunk64) ..; d.fdel the same for every IF.
+// 64 (g mod nchunk64) .. (those past the selection formed, not written);
+// d.fdel the same for every IF.
 // PREC as k_dft_split's: 0 hardware sin / cos for every phasor, 1 hardware
 // for the centre phasors P and the polynomial for the steps W and V, 2 the
 // polynomial for all.
@@ -562,6 +563,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
           for (int r = 0; r < 4; ++r) {
             const int c1 = 4 * (lane >> 4) + r;
             const int c = c1 + 8 * c2;
+            if (cbase + c >= d.nchsel) continue;  // past a ragged selection's last channel
             const double re = t ? o1[r] : o0[r], imv = t ? im1[r] : im0[r];
             if (pp) pp[c] = make_double2(re * unscale, imv * unscale);
             else apply_op(d, p + c * d.incf, (float)(re * unscale), (float)(imv * unscale));
@@ -1074,12 +1076,13 @@ int make_desc(State *st, const GPUInfo *gi, float *vis, long long nrow, int cpt,
   return kOK;
 }
 
-// Whether k_dft_mfma can run this descriptor: point components, whole
-// 64-channel groups and one channel step for every IF (within 1e-12; the
-// kernel uses the first IF's).  make_desc has left the per-IF lines in
-// st->hfreq.
+// Whether k_dft_mfma can run this descriptor: point components and one
+// channel step for every IF (within 1e-12; the kernel uses the first
+// IF's).  A selection that is not whole 64-channel groups runs its last
+// group in full and writes only the selected channels.  make_desc has left
+// the per-IF lines in st->hfreq.
 bool channels_ok(const State *st, const Desc &d) {
-  if (d.nchsel % 64 != 0 || !st->hfreq) return false;
+  if (d.nchsel < 1 || !st->hfreq) return false;
   const double d0 = st->hfreq[d.nIFsel];
   for (int i = 1; i < d.nIFsel; ++i)
     if (std::fabs(st->hfreq[d.nIFsel + i] - d0) > 1e-12 * std::fabs(d0)) return false;
@@ -1126,7 +1129,7 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
       MfmaFn fn = mfma_fn(st);
       const long long grid =
           std::min<long long>(nquad * std::max(d.nsplit, 1), (long long)st->nCU * occupancy_of(st, (const void *)fn));
-      const int nchunk64 = d.nchsel / 64;
+      const int nchunk64 = (d.nchsel + 63) / 64;
       hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
                          stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64, st->ampShift);
       OBIT_HIP_TRY(hipGetLastError());
@@ -1136,7 +1139,7 @@ int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipSt
       const long long nquad = (d.nrow + 3) / 4;
       MfmaFn fn = mfma_fn(st);
       const long long grid = std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
-      const int nchunk64 = d.nchsel / 64;
+      const int nchunk64 = (d.nchsel + 63) / 64;
       Desc dp = d;
       dp.cx = st->mx;
       dp.cy = st->my;

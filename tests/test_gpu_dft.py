@@ -515,14 +515,38 @@ def test_mixed_table_subtract(oracle):
         assert not np.array_equal(da[..., :2], db[..., :2])
 
 
-def test_matrix_core_kernel_falls_back(oracle):
-    """A channel selection that is not whole 64-channel groups (48 of 64)
-    and a Gaussian model: the cpt 64 variant runs the VALU kernel, with the
-    same result as cpt 32."""
+def test_matrix_core_ragged_channels(oracle):
+    """A channel selection that is not whole 64-channel groups (channels
+    9..56 of each IF's 64): the matrix-unit kernel runs each IF's 48
+    channels as one group and writes only those; replace against the FP64
+    oracle and the VALU kernel, subtract against the VALU kernel, and the
+    channels outside the selection untouched."""
     uv, sky, _ = synth.workload(2, rows=300, ncomp=300)
-    a = _run(uv, sky, op="repl", variant=(64, 64, 2), startChannel=9, numberChannel=48)
-    b = _run(uv, sky, op="repl", variant=(32, 64, 2), startChannel=9, numberChannel=48)
-    assert np.array_equal(a, b)
+    kw = dict(startChannel=9, numberChannel=48)
+    a = _run(uv, sky, op="repl", variant=(64, 64, 2), **kw)
+    b = _run(uv, sky, op="repl", variant=(32, 64, 2), **kw)
+    assert not np.array_equal(a, b)  # the matrix-unit kernel ran
+    d0 = uv.buffer[:, uv.nrparm:].reshape(uv.nrow, uv.nif, uv.nchan, uv.nstok, 3)
+    da = a[:, uv.nrparm:].reshape(d0.shape)
+    db = b[:, uv.nrparm:].reshape(d0.shape)
+    sel = np.zeros(uv.nchan, bool)
+    sel[8:56] = True  # startChannel is 1-relative
+    assert np.array_equal(da[:, :, ~sel], d0[:, :, ~sel])
+    ref = oracle.dft_model_fp64(uv, sky)
+    peak = np.abs(ref).max()
+    got = da[..., 0, 0] + 1j * da[..., 0, 1]
+    err = np.abs(got - ref)[:, :, sel].max() / peak
+    print(f"ragged channels: max {err:.3e}", flush=True)
+    assert err <= TOL
+    assert np.abs(da[..., :2] - db[..., :2]).max() <= 2 * TOL * peak
+    s1 = _run(uv, sky, op="sub", variant=(64, 64, 2), **kw)
+    s2 = _run(uv, sky, op="sub", variant=(32, 64, 2), **kw)
+    assert np.abs(s1[:, uv.nrparm:] - s2[:, uv.nrparm:]).max() <= 2 * TOL * peak + 1e-6 * np.abs(uv.buffer).max()
+
+
+def test_matrix_core_kernel_falls_back(oracle):
+    """A Gaussian model with replace: the cpt 64 variant runs the VALU
+    kernel, with the same result as cpt 32."""
     uv3, sky3, _ = synth.workload(3, rows=64)
     a = _run(uv3, sky3, op="repl", variant=(64, 64, 2))
     b = _run(uv3, sky3, op="repl", variant=(32, 64, 2))
