@@ -1240,8 +1240,9 @@ struct BufPlan {
 // with five equal segments, 4.9e12 with three, and 5.3-5.45e12 with 12,
 // 23 or 32 parts.
 //
-// Where k_dft_mfma can run the buffer (mfma_ok) it does, with its own
-// items: (component part, row quad).  The component list is cut into parts
+// Where k_dft_mfma can run the buffer (mfma_ok) and fills at least 0.6 of
+// its channel slots (whole groups of 64 channels, 4 a wave) it does, with
+// its own items: (component part, row quad).  The component list is cut into parts
 // of whole 128-component groups (the kernel's fold interval) so that the
 // buffer makes about 10 rounds of its resident workgroups, and segments
 // are sized as above with the middle ones a sixth of the buffer.
@@ -1253,7 +1254,11 @@ BufPlan plan_buffer(State *st, const Desc &d, size_t rowBytes) {
   (void)rowBytes;
   BufPlan p;
   p.kpart = std::max(d.ncomp, 1);
-  if (mfma_ok(st, d)) {
+  // k_dft_mfma forms 4 groups of 64 channels a wave: taken here where at
+  // least 0.6 of those slots are selected channels
+  const int ngrp = d.nIFsel * ((d.nchsel + 63) / 64);
+  const double fill = (double)d.nIFsel * d.nchsel / (256.0 * ((ngrp + 3) / 4));
+  if (mfma_ok(st, d) && fill >= 0.6) {
     p.cpt = 64;
     const long long slots = (long long)std::max(st->nCU, 1) * occupancy_of(st, (const void *)mfma_fn(st));
     const long long nquad = (d.nrow + 3) / 4;
