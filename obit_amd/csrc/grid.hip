@@ -348,22 +348,52 @@ __global__ void k_bin_count(const float *vis, long long nrow, const FacetDev *fa
 // scanned offset.  Sorting the pairs by key gives each tile's list ordered
 // by the window its tracks enter first, then by row (the sort is stable
 // and the emission ordered).
-__global__ void k_bin_emit(const float *vis, long long nrow, const FacetDev *facets, int nf, Common cm,
-                           const unsigned long long *off, unsigned *keys, int2 *rows) {
+// The entries of a block's 256 (row, facet) tracks are one contiguous
+// range of the output ([off[it0], off[it1])): where it fits, the block
+// stages it in LDS and writes it out whole (the tracks' own runs are short
+// and scattered); a larger range is written run by run.
+constexpr int kEmitCap = 4096;  // entries a block stages (48 KB)
+
+__global__ __launch_bounds__(256) void k_bin_emit(const float *vis, long long nrow, const FacetDev *facets,
+                                                  int nf, Common cm, const unsigned long long *off,
+                                                  unsigned *keys, int2 *rows) {
+  __shared__ unsigned s_key[kEmitCap];
+  __shared__ int2 s_row[kEmitCap];
   const long long total = nrow * (long long)nf;
-  for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
-       it += (long long)gridDim.x * blockDim.x) {
-    const int fi = (int)(it % nf);
-    const long long r = it / nf;
-    const FacetDev &f = facets[fi];
-    float uu, vv, ww;
-    rotate_uvw(f, vis + r * (long long)cm.lenvis, uu, vv, ww);
-    unsigned long long o = off[it];
-    track_tiles(f, cm, uu, vv, [&](long long L, unsigned wkey, int lo, int hi) {
-      keys[o] = ((unsigned)L << 7) | wkey;
-      rows[o] = make_int2((int)r, lo | (hi << 16));
-      ++o;
-    });
+  const long long nchunk = (total + 255) / 256;
+  for (long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+    const long long it0 = ch * 256, it1 = min(total, it0 + 256);
+    const unsigned long long base = off[it0], span = off[it1] - base;
+    const bool stage = span <= (unsigned long long)kEmitCap;  // the same for the whole block
+    const long long it = it0 + threadIdx.x;
+    if (it < total) {
+      const int fi = (int)(it % nf);
+      const long long r = it / nf;
+      const FacetDev &f = facets[fi];
+      float uu, vv, ww;
+      rotate_uvw(f, vis + r * (long long)cm.lenvis, uu, vv, ww);
+      unsigned long long o = off[it];
+      track_tiles(f, cm, uu, vv, [&](long long L, unsigned wkey, int lo, int hi) {
+        const unsigned k = ((unsigned)L << 7) | wkey;
+        const int2 v = make_int2((int)r, lo | (hi << 16));
+        if (stage) {
+          s_key[o - base] = k;
+          s_row[o - base] = v;
+        } else {
+          keys[o] = k;
+          rows[o] = v;
+        }
+        ++o;
+      });
+    }
+    if (stage) {
+      __syncthreads();
+      for (int j = threadIdx.x; j < (int)span; j += 256) {
+        keys[base + j] = s_key[j];
+        rows[base + j] = s_row[j];
+      }
+      __syncthreads();
+    }
   }
 }
 
