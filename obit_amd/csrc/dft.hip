@@ -1121,43 +1121,50 @@ int occupancy_of(State *st, const void *fn) {
 // the items; wgPerCU = 0: one workgroup per item, placed by the dispatcher
 // as CUs free up.  cpt = 64 selects k_dft_mfma where mfma_ok allows it (a
 // persistent grid of the resident workgroups over row quads), else 32.
+// k_dft_mfma over d's rows and components (its own items: component
+// parts x row quads), a persistent grid of its resident workgroups.  The
+// caller has checked the channels (channels_ok) and that d's components are
+// points.
+int launch_mfma(State *st, const Desc &d, hipStream_t stream) {
+  const long long nquad = (d.nrow + 3) / 4;
+  MfmaFn fn = mfma_fn(st);
+  const long long grid =
+      std::min<long long>(nquad * std::max(d.nsplit, 1), (long long)st->nCU * occupancy_of(st, (const void *)fn));
+  const int nchunk64 = (d.nchsel + 63) / 64;
+  hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
+                     stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64, st->ampShift);
+  OBIT_HIP_TRY(hipGetLastError());
+  return kOK;
+}
+
+// The points-first copy of a Gaussian table (SetMod): its points and its
+// Gaussians as two descriptors over d's rows.
+void mixed_descs(const State *st, const Desc &d, Desc &dp, Desc &dg) {
+  dp = d;
+  dp.cx = st->mx;
+  dp.cy = st->my;
+  dp.cz = st->mz;
+  dp.camp = st->mamp;
+  dp.ncomp = st->npoint;
+  dg = d;
+  dg.cx = st->mx + st->npoint;
+  dg.cy = st->my + st->npoint;
+  dg.cz = st->mz + st->npoint;
+  dg.camp = st->mamp + st->npoint;
+  dg.cg = st->cgG;
+  dg.ncomp = d.ncomp - st->npoint;
+  dg.nchunk = (d.nchsel + 31) / 32;
+}
+
 int launch_split(State *st, const Desc &d, int cpt, int tile, int wgPerCU, hipStream_t stream = nullptr) {
   if (d.nitems <= 0) return kOK;
   if (cpt == 64) {
-    if (mfma_ok(st, d)) {
-      const long long nquad = (d.nrow + 3) / 4;
-      MfmaFn fn = mfma_fn(st);
-      const long long grid =
-          std::min<long long>(nquad * std::max(d.nsplit, 1), (long long)st->nCU * occupancy_of(st, (const void *)fn));
-      const int nchunk64 = (d.nchsel + 63) / 64;
-      hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
-                         stream ? stream : st->stream, d, d.nIFsel * nchunk64, nchunk64, st->ampShift);
-      OBIT_HIP_TRY(hipGetLastError());
-      return kOK;
-    }
+    if (mfma_ok(st, d)) return launch_mfma(st, d, stream);
     if (mixed_ok(st, d)) {
-      const long long nquad = (d.nrow + 3) / 4;
-      MfmaFn fn = mfma_fn(st);
-      const long long grid = std::min<long long>(nquad, (long long)st->nCU * occupancy_of(st, (const void *)fn));
-      const int nchunk64 = (d.nchsel + 63) / 64;
-      Desc dp = d;
-      dp.cx = st->mx;
-      dp.cy = st->my;
-      dp.cz = st->mz;
-      dp.camp = st->mamp;
-      dp.ncomp = st->npoint;
-      hipLaunchKernelGGL(fn, dim3((unsigned)std::max<long long>(grid, 1)), dim3(kRows), 0,
-                         stream ? stream : st->stream, dp, d.nIFsel * nchunk64, nchunk64, st->ampShift);
-      OBIT_HIP_TRY(hipGetLastError());
-      if (st->npoint == d.ncomp) return kOK;
-      Desc dg = d;
-      dg.cx = st->mx + st->npoint;
-      dg.cy = st->my + st->npoint;
-      dg.cz = st->mz + st->npoint;
-      dg.camp = st->mamp + st->npoint;
-      dg.cg = st->cgG;
-      dg.ncomp = d.ncomp - st->npoint;
-      dg.nchunk = (d.nchsel + 31) / 32;
+      Desc dp, dg;
+      mixed_descs(st, d, dp, dg);
+      int rc = launch_mfma(st, dp, stream);
+      if (rc || st->npoint == d.ncomp) return rc;
       dg.nitems = (long long)d.nrowblk * d.nIFsel * dg.nchunk;
       return launch_split(st, dg, 32, tile, wgPerCU, stream);
     }
@@ -1216,6 +1223,11 @@ int occupancy(State *st, KernelFn fn) { return occupancy_of(st, (const void *)fn
 
 struct BufPlan {
   int cpt = 32, tile = 64, nsplit = 1, kpart = 1;
+  // a Gaussian table's points on k_dft_mfma (nsplit parts of kpart) and
+  // its Gaussians on the VALU kernel (nsplitG parts of kpartG), all parts'
+  // sums added by k_dft_combine
+  bool mixed = false;
+  int nsplitG = 0, kpartG = 1;
   bool serial = false;      // each segment's kernel waits for the one before
   std::vector<int> segBlk;  // row blocks of each segment
 };
@@ -1258,16 +1270,34 @@ BufPlan plan_buffer(State *st, const Desc &d, size_t rowBytes) {
   // least 0.6 of those slots are selected channels
   const int ngrp = d.nIFsel * ((d.nchsel + 63) / 64);
   const double fill = (double)d.nIFsel * d.nchsel / (256.0 * ((ngrp + 3) / 4));
-  if (mfma_ok(st, d) && fill >= 0.6) {
+  const double chans = (double)d.nrow * d.nIFsel * d.nchsel;
+  const double capParts = double(size_t(1) << 30) / (chans * sizeof(double2));  // parts the scratch holds
+  const bool mixed = st->type == GPUModTypeGauss && st->npoint > 0 && st->npoint < d.ncomp &&
+                     channels_ok(st, d) && fill >= 0.6 && capParts >= 2.0;
+  if ((mfma_ok(st, d) && fill >= 0.6) || mixed) {
     p.cpt = 64;
     const long long slots = (long long)std::max(st->nCU, 1) * occupancy_of(st, (const void *)mfma_fn(st));
     const long long nquad = (d.nrow + 3) / 4;
-    if (d.ncomp > 0) {
-      const double chans = (double)d.nrow * d.nIFsel * d.nchsel;
+    const int npt = mixed ? st->npoint : d.ncomp;
+    if (mixed) {
+      // the Gaussians: VALU items of whole 64-component tiles, about 10
+      // rounds of that kernel's resident workgroups
+      const int ng = d.ncomp - st->npoint;
+      const long long slotsV =
+          (long long)std::max(st->nCU, 1) * occupancy(st, pick_split(32, p.tile, true, false, st->prec));
+      const long long perPart = (long long)d.nrowblk * d.nIFsel * ((d.nchsel + 31) / 32);
+      int q = (int)std::min<long long>(16, std::max<long long>(1, (10 * slotsV + perPart - 1) / perPart));
+      q = std::min(q, std::max(1, (int)capParts - 1));
+      p.kpartG = ((ng + q - 1) / q + p.tile - 1) / p.tile * p.tile;
+      p.nsplitG = (ng + p.kpartG - 1) / p.kpartG;
+      p.mixed = true;
+      p.kpart = std::max(npt, 1);
+    }
+    if (npt > 0) {
       const int want = (int)std::min<long long>(16, std::max<long long>(1, (10 * slots + nquad - 1) / nquad));
-      const int kpt = ((d.ncomp + want - 1) / want + 127) / 128 * 128;
-      const int ns = (d.ncomp + kpt - 1) / kpt;
-      if (ns > 1 && (double)ns * chans * sizeof(double2) <= double(size_t(1) << 30)) {
+      const int kpt = ((npt + want - 1) / want + 127) / 128 * 128;
+      const int ns = (npt + kpt - 1) / kpt;
+      if (ns > 1 && ns + p.nsplitG <= capParts) {
         p.nsplit = ns;
         p.kpart = kpt;
       }
@@ -1587,9 +1617,10 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
   }
   const BufPlan plan = plan_buffer(st, d, rowBytes);
   const int cpt = plan.cpt, nsplit = plan.nsplit, nseg = (int)plan.segBlk.size();
+  const int ntot = nsplit + plan.nsplitG;  // parts in the scratch
   d.nchunk = (d.nchsel + cpt - 1) / cpt;
-  if (nsplit > 1) {
-    const size_t need = (size_t)nsplit * (size_t)nvis * d.nIFsel * d.nchsel * sizeof(double2);
+  if (ntot > 1) {
+    const size_t need = (size_t)ntot * (size_t)nvis * d.nIFsel * d.nchsel * sizeof(double2);
     if (need > st->partCap) {
       if (st->part) OBIT_HIP_CHECK(hipFree(st->part));
       st->part = nullptr;
@@ -1600,7 +1631,7 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
   }
   st->lastPath = 0;
   st->lastCpt = cpt;
-  st->lastSplit = nsplit;
+  st->lastSplit = ntot;
   // The buffer in row segments on the caller's streams (ObitGPUSkyModel-
   // DFTInit creates them), round robin: each segment's upload, launch and
   // download queue on its stream, so kernels run while later segments are
@@ -1642,10 +1673,23 @@ void ObitCUDASkyModelDFTCalc(GPUInfo *gpuInfo) {
     ds.nrowblk = (int)((n + kRows - 1) / kRows);
     ds.nsplit = nsplit;
     ds.kpart = plan.kpart;
-    ds.part = nsplit > 1 ? st->part + (size_t)nsplit * (size_t)r0 * d.nIFsel * d.nchsel : nullptr;
+    ds.part = ntot > 1 ? st->part + (size_t)ntot * (size_t)r0 * d.nIFsel * d.nchsel : nullptr;
     ds.nitems = (long long)ds.nrowblk * ds.nIFsel * ds.nchunk * nsplit;
-    rc = launch_split(st, ds, cpt, plan.tile, 0, s);
-    if (!rc && nsplit > 1) {
+    if (plan.mixed) {
+      // points into parts 0 .. nsplit-1, Gaussians into the next nsplitG
+      Desc dp, dg;
+      mixed_descs(st, ds, dp, dg);
+      rc = launch_mfma(st, dp, s);
+      dg.nsplit = plan.nsplitG;
+      dg.kpart = plan.kpartG;
+      dg.part = ds.part + (size_t)nsplit * (size_t)n * d.nIFsel * d.nchsel;
+      dg.nitems = (long long)dg.nrowblk * dg.nIFsel * dg.nchunk * plan.nsplitG;
+      if (!rc) rc = launch_split(st, dg, 32, plan.tile, 0, s);
+      ds.nsplit = ntot;
+    } else {
+      rc = launch_split(st, ds, cpt, plan.tile, 0, s);
+    }
+    if (!rc && ntot > 1) {
       const long long m = n * ds.nIFsel * ds.nchsel;
       const int nb = (int)std::min<long long>((m + 255) / 256, (long long)st->nCU * 8);
       hipLaunchKernelGGL(k_dft_combine, dim3(nb), dim3(256), 0, s, ds);

@@ -372,7 +372,8 @@ def test_buffer_component_parts(oracle, kind):
     """Small per-buffer pieces cut the component list into parts whose FP64
     sums k_dft_combine adds and applies: the model (replace) against the
     FP64 oracle, and subtract / divide against the resident launch, for
-    point, Gaussian and spectral types; the last piece is ragged."""
+    point, Gaussian (points and Gaussians in parts on their own kernels)
+    and spectral types; the last piece is ragged."""
     from obit_amd.skymodel import SkyModelDFT
     uv, sky0, _ = synth.workload(2, rows=1100, ncomp=2000)
     sky = _split_sky(kind, uv, sky0, np.random.default_rng(31))
@@ -381,6 +382,10 @@ def test_buffer_component_parts(oracle, kind):
         cpt, nsplit = sm.buffer_shape()
     print(f"{kind}: {cpt} channels per thread, {nsplit} component parts", flush=True)
     assert nsplit > 1
+    if kind in ("point", "gauss"):
+        # points on the matrix-unit kernel (a Gaussian table's points too,
+        # its Gaussians in their own parts on the VALU kernel)
+        assert cpt == 64
     ref = oracle.dft_model_fp64(uv, sky)
     peak = np.abs(ref).max()
     err = np.abs(_model_from(uv, out) - ref).max() / peak
